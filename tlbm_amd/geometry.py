@@ -1,0 +1,160 @@
+"""Node classification (ndType) in integer lattice units.
+
+The reference's pre-processing (python/FluidChannel.py) builds geometry in
+physical float32 coordinates and classifies nodes by these rules, which are
+kept here on the integer lattice (x, y, z are node indices):
+
+* walls (FluidChannel.py:1517-1538): 'right' x == 0, 'left' x == nx-1,
+  'bottom' y == 0, 'top' y == ny-1 -> solid;
+* obstacles: sphere and crossflow cylinder strictly inside (d^2 < r^2,
+  FluidChannel.py:266-276, 411-416), prolate spheroid on or inside
+  (g <= 0, FluidChannel.py:378-393);
+* inlet = plane z == 0 minus walls and obstacles -> 2, outlet = plane
+  z == nz-1 minus walls and obstacles -> 3 (FluidChannel.py:1358-1397);
+* solid = obstacles union walls -> 1, fluid -> 0;
+* lid-driven cavity (FluidChannel.py:1179-1243): walls x == 0, x == nx-1,
+  y == 0, z == 0, z == nz-1 solid, then the lid y == ny-1 -> 5.
+
+Arrays are returned flat in the reference's global order
+g = x + nx*y + nx*ny*z (TLBM_Partition.cpp:730-737) as uint8.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+FLUID, SOLID, INLET, OUTLET, LID = 0, 1, 2, 3, 5
+VALID_TYPES = (FLUID, SOLID, INLET, OUTLET, LID)
+
+
+def grid(nx: int, ny: int, nz: int):
+    """Index grids shaped (nz, ny, nx), so ravel() is global order."""
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    return x, y, z
+
+
+def sphere(xc, yc, zc, r):
+    def mask(x, y, z):
+        return (x - xc) ** 2 + (y - yc) ** 2 + (z - zc) ** 2 < r * r
+    return mask
+
+
+def cylinder_x(yc, zc, r):
+    """Crossflow cylinder with its axis along x."""
+    def mask(x, y, z):
+        return (y - yc) ** 2 + (z - zc) ** 2 < r * r
+    return mask
+
+
+def prolate_spheroid(xc, yc, zc, ab, c):
+    """Semi-axis ab across the flow (x, y) and c along z, zero incidence."""
+    def mask(x, y, z):
+        dx = (x - xc).astype(np.float64)
+        dy = (y - yc).astype(np.float64)
+        dz = (z - zc).astype(np.float64)
+        g = (dx * dx + dy * dy) / float(ab * ab) + (dz * dz) / float(c * c) - 1.0
+        return g <= 0.0
+    return mask
+
+
+def channel(nx: int, ny: int, nz: int, walls=("left", "right", "top", "bottom"),
+            obstacles=(), inlet: bool = True, outlet: bool = True) -> np.ndarray:
+    """Channel along z with optional side walls, obstacles, inlet and outlet."""
+    x, y, z = grid(nx, ny, nz)
+    wall = np.zeros(x.shape, dtype=bool)
+    for w in walls:
+        if w == "right":
+            wall |= x == 0
+        elif w == "left":
+            wall |= x == nx - 1
+        elif w == "bottom":
+            wall |= y == 0
+        elif w == "top":
+            wall |= y == ny - 1
+        else:
+            raise ValueError(f"unknown wall {w!r}")
+    obst = np.zeros(x.shape, dtype=bool)
+    for m in obstacles:
+        obst |= m(x, y, z)
+    nd = np.zeros(x.shape, dtype=np.uint8)
+    nd[wall | obst] = SOLID
+    free = ~(wall | obst)
+    if inlet:
+        nd[(z == 0) & free] = INLET
+    if outlet:
+        nd[(z == nz - 1) & free] = OUTLET
+    return nd.ravel()
+
+
+def lid_cavity(nx: int, ny: int, nz: int) -> np.ndarray:
+    x, y, z = grid(nx, ny, nz)
+    nd = np.zeros(x.shape, dtype=np.uint8)
+    nd[(x == 0) | (x == nx - 1) | (y == 0) | (z == 0) | (z == nz - 1)] = SOLID
+    nd[y == ny - 1] = LID
+    return nd.ravel()
+
+
+def periodic_box(nx: int, ny: int, nz: int, obstacles=()) -> np.ndarray:
+    """All-fluid periodic box, optionally with solid obstacles."""
+    return channel(nx, ny, nz, walls=(), obstacles=obstacles, inlet=False, outlet=False)
+
+
+def rsa_spheres(nx: int, ny: int, nz: int, r_min: float, r_max: float, porosity: float,
+                seed: int, max_attempts: int = 2_000_000, walls=True):
+    """Seeded random sequential addition of non-overlapping spheres.
+
+    Radii are uniform in [r_min, r_max], centres uniform in the box; a
+    candidate is rejected if it overlaps an accepted sphere.  Spheres are
+    added until the solid fraction of the obstacle mask reaches
+    1 - porosity or max_attempts candidates have been drawn.  A cell list
+    keyed on the largest diameter keeps the overlap test local.
+    Returns (centres, radii) as float64 arrays.
+    """
+    rng = np.random.default_rng(seed)
+    cell = 2.0 * r_max
+    ncx, ncy, ncz = (max(1, int(np.ceil(n / cell))) for n in (nx, ny, nz))
+    cells: dict = {}
+    centres, radii = [], []
+    target = (1.0 - porosity) * nx * ny * nz
+    solid = 0.0
+    attempts = 0
+    while solid < target and attempts < max_attempts:
+        attempts += 1
+        r = rng.uniform(r_min, r_max)
+        c = rng.uniform((0.0, 0.0, 0.0), (nx, ny, nz))
+        ci = (min(int(c[0] / cell), ncx - 1), min(int(c[1] / cell), ncy - 1),
+              min(int(c[2] / cell), ncz - 1))
+        ok = True
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    for j in cells.get((ci[0] + dx, ci[1] + dy, ci[2] + dz), ()):
+                        d2 = float(np.sum((centres[j] - c) ** 2))
+                        if d2 < (radii[j] + r) ** 2:
+                            ok = False
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if not ok:
+                break
+        if not ok:
+            continue
+        cells.setdefault(ci, []).append(len(centres))
+        centres.append(c)
+        radii.append(r)
+        solid += 4.0 / 3.0 * np.pi * r ** 3
+    return np.array(centres, dtype=np.float64).reshape(-1, 3), np.array(radii, dtype=np.float64)
+
+
+def sphere_pack_mask(nx, ny, nz, centres, radii):
+    """Obstacle mask for a sphere pack, evaluated per sphere bounding box."""
+    mask = np.zeros((nz, ny, nx), dtype=bool)
+    for (cx, cy, cz), r in zip(centres, radii):
+        x0, x1 = max(0, int(np.floor(cx - r))), min(nx, int(np.ceil(cx + r)) + 1)
+        y0, y1 = max(0, int(np.floor(cy - r))), min(ny, int(np.ceil(cy + r)) + 1)
+        z0, z1 = max(0, int(np.floor(cz - r))), min(nz, int(np.ceil(cz + r)) + 1)
+        zz, yy, xx = np.meshgrid(np.arange(z0, z1), np.arange(y0, y1), np.arange(x0, x1),
+                                 indexing="ij")
+        mask[z0:z1, y0:y1, x0:x1] |= (xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2 < r * r
+    return mask
