@@ -1,0 +1,121 @@
+/*
+ * tlbm_gpu.h -- C-ABI of the MI355X lattice-Boltzmann time step.
+ *
+ * One handle holds one whole lattice on one GPU (the reference's
+ * TLBM_Partition for a single partition).  Plain pointers and sizes only;
+ * host arrays are in the reference's global node order
+ * g = x + nx*y + nx*ny*z (TLBM_Partition.cpp:730-737).
+ *
+ * Every function returns TLBM_OK (0) or a negative TLBM_E* code and leaves
+ * a message for tlbm_gpu_last_error() on the calling thread.  Steps are
+ * queued on the handle's HIP stream and return before the GPU finishes;
+ * every function that hands results to the host synchronises that stream.
+ * A handle is used by one host thread at a time.
+ */
+#ifndef TLBM_GPU_H
+#define TLBM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TLBM_OK 0
+#define TLBM_EINVAL (-1)   /* bad argument: lattice, size, node type, mode */
+#define TLBM_EHIP (-2)     /* a HIP runtime call failed */
+#define TLBM_ENOMEM (-3)   /* device or host allocation failed */
+#define TLBM_ESTATE (-4)   /* call not valid in the handle's current state */
+
+/* Population storage in HBM.  F64 stores f_s and computes in FP64 with the
+ * reference's operation order.  F32 stores d_s = f_s - w_s and F16 the
+ * scaled deviation (f_s - w_s)/w_s; both compute in FP32 on d_s. */
+#define TLBM_STORAGE_F64 0
+#define TLBM_STORAGE_F32 1
+#define TLBM_STORAGE_F16 2
+
+typedef struct tlbm_gpu tlbm_gpu;
+
+/* The fields of Problem (include/Problem.h:10-45) the time step reads,
+ * plus the storage choice. */
+typedef struct tlbm_params {
+    int lattice;       /* 15, 19 or 27: Problem::latticeType "D3Q15"/"D3Q19"/"D3Q27" */
+    int dynamics;      /* Problem::dynamics; 1 = BGK, any other value: no relaxation */
+    int nx, ny, nz;    /* Problem::nx, ny, nz */
+    int storage;       /* TLBM_STORAGE_* */
+    int exact_weights; /* 0: the reference's float-literal D3Q15/D3Q19 weights */
+    int reserved;      /* must be 0 */
+    double cs;         /* Problem::cs, Smagorinsky when > 0 */
+    double rho_lbm;    /* Problem::rhoLBM: initial and outlet density */
+    double u_lbm;      /* Problem::uLBM: inlet / prescribed u_z */
+    double omega;      /* Problem::omega */
+} tlbm_params;
+
+/* Allocate the lattice on the current device and set f = w*rho_lbm.
+ * nd_type: nx*ny*nz node types in {0,1,2,3,5}.
+ * Replaces TLBM_Partition::TLBM_Partition / tlbm_initialize
+ * (src/TLBM_Partition.cpp:10-14, 77-118) with load_ndType (120-141),
+ * create_adj_matrix (254-271), allocate_arrays (421-446) and
+ * initialize_data_arrays (448-479) on one partition.  Node types outside
+ * {0,1,2,3,5} are rejected (the reference relaxes them against a stale fEq),
+ * and a type-5 node with index below Q-1 is rejected (its uy[spd] term
+ * would read values the reference has not computed yet). */
+int tlbm_gpu_create(const tlbm_params *p, const uint8_t *nd_type, tlbm_gpu **out);
+
+/* Free every device buffer.  ~TLBM_Partition (src/TLBM_Partition.cpp:16-45). */
+int tlbm_gpu_destroy(tlbm_gpu *h);
+
+/* Queue nsteps time steps.  TLBM_Partition::take_LBM_time_step
+ * (src/TLBM_Partition.cpp:910-947) called nsteps times. */
+int tlbm_gpu_step(tlbm_gpu *h, int nsteps);
+
+/* Run nsteps steps between two HIP events on the handle's stream and
+ * return their elapsed time in milliseconds (synchronises). */
+int tlbm_gpu_step_timed(tlbm_gpu *h, int nsteps, double *ms);
+
+/* Wait for queued work. */
+int tlbm_gpu_sync(tlbm_gpu *h);
+
+/* ux, uy, uz, rho of the last step, as TLBM_Partition::write_data dumps
+ * them (src/TLBM_Partition.cpp:608-650): moments after the boundary rule
+ * (solid u = 0, inlet u = (0,0,u_lbm), outlet rho = rho_lbm).  Each array
+ * holds nx*ny*nz doubles.  Needs at least one step since create/set. */
+int tlbm_gpu_get_macros(tlbm_gpu *h, double *ux, double *uy, double *uz, double *rho);
+
+/* Momentum-exchange force on solid nodes next to fluid nodes, after the
+ * last step.  TLBM_Partition::make_force_calc_map + calc_force
+ * (src/TLBM_Partition.cpp:143-208) on one partition. */
+int tlbm_gpu_get_forces(tlbm_gpu *h, double *fx, double *fy, double *fz);
+
+/* Populations the next step will read, node-major f[g*Q + s]: the
+ * reference's fOut after the last step, read one member at a time by
+ * TLBM_Partition::get_data_member (src/TLBM_Partition.cpp:860-864). */
+int tlbm_gpu_get_populations(tlbm_gpu *h, double *f);
+
+/* Set the populations the next step will read (node-major f[g*Q + s]).
+ * TLBM_Partition::set_data_member (src/TLBM_Partition.cpp:866-870) for
+ * every member at once. */
+int tlbm_gpu_set_populations(tlbm_gpu *h, const double *f);
+
+/* f = f_eq(rho, u) per node: TLBM_Partition::load_restart_data
+ * (src/TLBM_Partition.cpp:481-540) after the HDF5 read. */
+int tlbm_gpu_init_from_macros(tlbm_gpu *h, const double *ux, const double *uy, const double *uz,
+                              const double *rho);
+
+/* FP64 reductions over the populations the next step will read:
+ * out[0] = mass sum f, out[1..3] = momentum sum f*e, out[4] = relative
+ * change of the state over the last step, ||B_n - B_{n-1}|| / ||B_n||
+ * (0 before the first step).  No reference counterpart: the reference has
+ * no conservation or residual checks. */
+int tlbm_gpu_reduce(tlbm_gpu *h, double out[5]);
+
+/* Bytes of device memory the handle holds. */
+int tlbm_gpu_memory_bytes(const tlbm_gpu *h, uint64_t *bytes);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *tlbm_gpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -9,6 +9,13 @@
 #include <string.h>
 
 #include "lbm_oracle.h"
+
+/* ORACLE_PLANT selects one deliberately wrong variant of the restatement
+ * (tests/test_oracle.py builds them to show the golden fixtures catch each
+ * mistake); 0, the default, is the faithful restatement. */
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
 #include "oracle_lattice.h"
 
 #define REAL double

@@ -63,6 +63,11 @@ void oracle_get_forces_f32(const oracle_f32 *o, float *fx, float *fy, float *fz)
 void oracle_get_populations_f64(const oracle_f64 *o, double *f);
 void oracle_get_populations_f32(const oracle_f32 *o, float *f);
 
+/* Set the populations the next step reads (node-major f[g*Q + s]) in both
+ * buffers: set_data_member (TLBM_Partition.cpp:866-870) for every member. */
+void oracle_set_populations_f64(oracle_f64 *o, const double *f);
+void oracle_set_populations_f32(oracle_f32 *o, const float *f);
+
 /* Weight table the oracle uses (Q entries), for table checks. */
 void oracle_get_weights_f64(const oracle_f64 *o, double *w);
 

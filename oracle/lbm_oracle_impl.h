@@ -32,8 +32,12 @@ static void FN(equilibrium)(struct FN(oracle) *o, long nd)
     REAL cu;
     for (int s = 0; s < o->Q; ++s) {
         cu = 3.0 * (L->ex[s] * o->ux[nd] + L->ey[s] * o->uy[nd] + L->ez[s] * o->uz[nd]);
+#if ORACLE_PLANT == 1
+        feq[s] = o->w[s] * o->rho[nd] * (1.0 + cu -
+#else
         feq[s] = o->w[s] * o->rho[nd] *
                  (1.0 + cu + 0.5 * (cu * cu) -
+#endif
                   3. / 2. * (o->ux[nd] * o->ux[nd] + o->uy[nd] * o->uy[nd] + o->uz[nd] * o->uz[nd]));
     }
 }
@@ -47,6 +51,9 @@ static REAL FN(turbulence)(REAL omega, const REAL *S, REAL cs)
     P = sqrt(S[0] * S[0] + S[4] * S[4] + S[8] * S[8] +
              2.0 * (S[1] * S[1] + S[2] * S[2] + S[5] * S[5]));
     P *= cs;
+#if ORACLE_PLANT == 5
+    P *= cs;
+#endif
     P = sqrt(P + nu * nu) - nu;
     nu_e = P / 6.;
     omega_r = 1. / (3. * (nu + nu_e) + 0.5);
@@ -102,8 +109,14 @@ static void FN(process_node)(struct FN(oracle) *o, REAL *fOut, REAL *fIn, long n
             else
                 o->rho[nd] = (1. / (1. - u_bc)) * (2. * s_in + sz);
             FN(equilibrium)(o, nd);
-            for (int i = 0; i < L->n_unk; ++i)
-                f[L->in_unk[i]] = feq[L->in_unk[i]] + f[L->in_par[i]] - feq[L->in_par[i]];
+            for (int i = 0; i < L->n_unk; ++i) {
+#if ORACLE_PLANT == 3
+                const int par = L->bb[L->in_unk[i]];
+#else
+                const int par = L->in_par[i];
+#endif
+                f[L->in_unk[i]] = feq[L->in_unk[i]] + f[par] - feq[par];
+            }
         }
         if (t == 3) {
             /* set_outletE_bc_macro / _micro; ux, uy stay from the moments */
@@ -112,7 +125,11 @@ static void FN(process_node)(struct FN(oracle) *o, REAL *fOut, REAL *fIn, long n
             o->rho[nd] = rho_bc;
             for (int i = 0; i < L->n_unk; ++i) sout += f[L->out_list[i]];
             for (int i = 0; i < L->n_zero; ++i) sz += f[L->zero_list[i]];
+#if ORACLE_PLANT == 4
+            o->uz[nd] = 1. - (1. / rho_bc) * (2. * sout + sz);
+#else
             o->uz[nd] = -1. + (1. / rho_bc) * (2. * sout + sz);
+#endif
             FN(equilibrium)(o, nd);
             for (int i = 0; i < L->n_unk; ++i)
                 f[L->out_unk[i]] = feq[L->out_unk[i]] + f[L->out_par[i]] - feq[L->out_par[i]];
@@ -124,7 +141,11 @@ static void FN(process_node)(struct FN(oracle) *o, REAL *fOut, REAL *fIn, long n
             for (int s = 0; s < Q; ++s)
                 f[s] += 3. * o->rho[nd] * o->w[s] *
                         (L->ez[s] * (u_bc - o->uz[nd]) + L->ex[s] * (0. - o->ux[nd]) +
+#if ORACLE_PLANT == 6
+                         L->ey[s] * (0. - o->uy[nd]));
+#else
                          L->ey[s] * (0. - o->uy[s]));
+#endif
             FN(equilibrium)(o, nd);
         }
 
@@ -198,7 +219,11 @@ struct FN(oracle) *FN(oracle_create)(const oracle_params *p, const int *ndType)
         const int y = (int)((g - (long)z * nx * ny) / nx);
         const int x = (int)(g - (long)z * nx * ny - (long)y * nx);
         for (int s = 0; s < Q; ++s) {
+#if ORACLE_PLANT == 2
+            int tx = (x + L->ex[s]) % nx, ty = (y - L->ey[s]) % ny, tz = (z + L->ez[s]) % nz;
+#else
             int tx = (x + L->ex[s]) % nx, ty = (y + L->ey[s]) % ny, tz = (z + L->ez[s]) % nz;
+#endif
             if (tx < 0) tx += nx;
             if (ty < 0) ty += ny;
             if (tz < 0) tz += nz;
@@ -276,4 +301,11 @@ void FN(oracle_get_populations)(const struct FN(oracle) *o, REAL *f)
 {
     const REAL *src = o->fOut ? o->fOut : o->fOdd;
     memcpy(f, src, sizeof(REAL) * o->N * o->Q);
+}
+
+void FN(oracle_set_populations)(struct FN(oracle) *o, const REAL *f)
+{
+    memcpy(o->fEven, f, sizeof(REAL) * o->N * o->Q);
+    memcpy(o->fOdd, f, sizeof(REAL) * o->N * o->Q);
+    o->fOut = (o->ts % 2) != 0 ? o->fEven : o->fOdd;
 }

@@ -40,6 +40,7 @@ def lib():
             for fn in ("get_macros", "get_forces", "get_populations"):
                 n = {"get_macros": 4, "get_forces": 3, "get_populations": 1}[fn]
                 getattr(L, f"oracle_{fn}_{suf}").argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * n
+            getattr(L, f"oracle_set_populations_{suf}").argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_get_weights_f64.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
     return _lib
@@ -88,6 +89,12 @@ class Oracle:
         f = np.empty(self.N * self.Q, self.dtype)
         getattr(lib(), f"oracle_get_populations_{self.suf}")(self._h, f.ctypes.data)
         return f.reshape(self.N, self.Q)
+
+    def set_populations(self, f) -> None:
+        a = np.ascontiguousarray(f, dtype=self.dtype).ravel()
+        if a.size != self.N * self.Q:
+            raise ValueError(f"expected {self.N * self.Q} populations, got {a.size}")
+        getattr(lib(), f"oracle_set_populations_{self.suf}")(self._h, a.ctypes.data)
 
     def weights(self):
         if self.suf != "f64":

@@ -1,0 +1,240 @@
+"""Parity of the HIP path (through the C-ABI) with the reference and the oracle.
+
+* FP64 storage reproduces every golden fixture of the compiled reference
+  (moments and forces) to 1e-12 relative, the tolerance the time step is
+  held to in double precision.
+* From random (non-uniform) states, one and several steps match the oracle
+  for every lattice and node type, so streaming, periodic wrap, bounce-back,
+  inlet/outlet, the type-5 rule and Smagorinsky are all exercised from the
+  first step.
+* FP32 storage: velocity L2 error against FP64 <= 1e-5; FP16 (f - w)
+  storage: <= 1e-3.
+* Restart (equilibrium init), populations, forces and the FP64 reductions.
+"""
+import numpy as np
+import pytest
+
+import goldens
+from pyoracle import Oracle
+from tlbm_amd import geometry
+from tlbm_amd.gpu import Lattice
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+CASES = [c for c in goldens.load_cases() if c.precision == "f64"]
+FIELDS = (("ux", "ux"), ("uy", "uy"), ("uz", "uz"), ("density", "rho"))
+
+
+def rel_err(got, ref):
+    scale = np.max(np.abs(ref))
+    if scale == 0:
+        return float(np.max(np.abs(got)))
+    return float(np.max(np.abs(got - ref)) / scale)
+
+
+def l2_rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c.name for c in CASES])
+def test_fp64_matches_reference_fixture(gpu_lib, case):
+    print(f"case {case.name} q={case.q} dims={case.dims}", flush=True)
+    with Lattice(case.q, *case.dims, case.nd, storage="f64", **case.kwargs()) as lat:
+        lat.step(case.params["numTs"])
+        m = lat.macros()
+        for field, key in FIELDS:
+            assert rel_err(m[key], case.out[field]) <= TOL64, field
+        if "Fx" in case.out:
+            fo = lat.forces()
+            for field in ("Fx", "Fy", "Fz"):
+                assert rel_err(fo[field], case.out[field]) <= TOL64, field
+
+
+def _random_state(rng, w, n, amp=0.05):
+    return w[None, :] * (1.0 + amp * rng.standard_normal((n, w.size)))
+
+
+GEOMS = {
+    "channel": lambda: (9, 7, 11, geometry.channel(9, 7, 11, obstacles=[geometry.sphere(4, 3, 5, 2)])),
+    "lid": lambda: (7, 8, 6, geometry.lid_cavity(7, 8, 6)),
+    "periodic": lambda: (5, 6, 4, geometry.periodic_box(5, 6, 4, obstacles=[geometry.sphere(2, 3, 2, 1.2)])),
+}
+
+
+@pytest.mark.parametrize("geom", sorted(GEOMS))
+@pytest.mark.parametrize("q", [15, 19, 27])
+@pytest.mark.parametrize("cs", [0.0, 0.125])
+def test_fp64_random_state_matches_oracle(gpu_lib, geom, q, cs):
+    nx, ny, nz, nd = GEOMS[geom]()
+    print(f"random state {geom} q={q} cs={cs}", flush=True)
+    kw = dict(omega=1.6, u_lbm=0.04, rho_lbm=1.0, cs=cs)
+    o = Oracle(q, nx, ny, nz, nd, **kw)
+    rng = np.random.default_rng(q * 100 + int(cs * 8))
+    f0 = _random_state(rng, o.weights(), nx * ny * nz)
+    o.set_populations(f0)
+    with Lattice(q, nx, ny, nz, nd, storage="f64", **kw) as lat:
+        lat.set_populations(f0)
+        assert np.array_equal(lat.populations(), f0)
+        for nsteps in (1, 4):
+            o.step(nsteps)
+            lat.step(nsteps)
+            mo, mg = o.macros(), lat.macros()
+            for key in ("ux", "uy", "uz", "rho"):
+                assert rel_err(mg[key], mo[key]) <= TOL64, (nsteps, key)
+            assert rel_err(lat.populations(), o.populations()) <= TOL64
+            fo, fg = o.forces(), lat.forces()
+            for key in ("Fx", "Fy", "Fz"):
+                assert rel_err(fg[key], fo[key]) <= TOL64, key
+
+
+def test_fp64_dynamics_other_than_bgk_does_not_relax(gpu_lib):
+    nx, ny, nz, nd = GEOMS["channel"]()
+    kw = dict(omega=1.6, u_lbm=0.04, rho_lbm=1.0, cs=0.1, dynamics=2)
+    o = Oracle(19, nx, ny, nz, nd, **kw)
+    f0 = _random_state(np.random.default_rng(3), o.weights(), nx * ny * nz)
+    o.set_populations(f0)
+    with Lattice(19, nx, ny, nz, nd, storage="f64", **kw) as lat:
+        lat.set_populations(f0)
+        o.step(3)
+        lat.step(3)
+        assert rel_err(lat.populations(), o.populations()) <= TOL64
+
+
+def test_fp64_is_deterministic(gpu_lib):
+    nx, ny, nz, nd = GEOMS["channel"]()
+    runs = []
+    for _ in range(2):
+        with Lattice(19, nx, ny, nz, nd, storage="f64", omega=1.7, u_lbm=0.05, cs=0.1) as lat:
+            lat.step(20)
+            runs.append(lat.populations())
+    assert np.array_equal(runs[0], runs[1])
+
+
+def _feq_reference_order(w, e, rho, ux, uy, uz):
+    """f_eq in the reference's operation order (LatticeStructure.hpp:92-103)."""
+    out = np.empty((rho.size, w.size))
+    for s in range(w.size):
+        cu = 3.0 * (e[s, 0] * ux + e[s, 1] * uy + e[s, 2] * uz)
+        out[:, s] = w[s] * rho * (1.0 + cu + 0.5 * (cu * cu) - 3.0 / 2.0 * (ux * ux + uy * uy + uz * uz))
+    return out
+
+
+@pytest.mark.parametrize("q", [15, 19, 27])
+def test_restart_init_is_equilibrium(gpu_lib, q):
+    nx, ny, nz = 6, 5, 7
+    n = nx * ny * nz
+    rng = np.random.default_rng(11)
+    rho = 1.0 + 0.01 * rng.standard_normal(n)
+    u = 0.05 * rng.standard_normal((3, n))
+    w = Oracle(q, nx, ny, nz, np.zeros(n, int), omega=1.2).weights()
+    e = _velocities(q)
+    want = _feq_reference_order(w, e, rho, *u)
+    with Lattice(q, nx, ny, nz, np.zeros(n, np.uint8), storage="f64", omega=1.2) as lat:
+        lat.init_from_macros(u[0], u[1], u[2], rho)
+        assert rel_err(lat.populations(), want) <= 1e-15
+
+
+def _velocities(q):
+    import os
+    import subprocess
+    import tempfile
+    src = ('#include <stdio.h>\n#include <stdlib.h>\n#include "oracle_lattice.h"\n'
+           'int main(int c, char **v){const oracle_lattice *L = oracle_lattice_for(atoi(v[1]));'
+           'for(int s=0;s<L->Q;++s) printf("%d %d %d\\n", L->ex[s], L->ey[s], L->ez[s]); return 0;}')
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    with tempfile.TemporaryDirectory() as d:
+        open(os.path.join(d, "p.c"), "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(root, "oracle"), os.path.join(d, "p.c"), "-o",
+                        os.path.join(d, "p")], check=True)
+        out = subprocess.run([os.path.join(d, "p"), str(q)], capture_output=True, text=True,
+                             check=True).stdout
+    return np.array([[int(v) for v in ln.split()] for ln in out.strip().splitlines()])
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
+def test_reductions_match_host_sums(gpu_lib, storage):
+    nx, ny, nz, nd = GEOMS["channel"]()
+    q = 19
+    e = _velocities(q)
+    with Lattice(q, nx, ny, nz, nd, storage=storage, omega=1.6, u_lbm=0.05) as lat:
+        r0 = lat.reduce()
+        assert r0["residual"] == 0.0
+        lat.step(7)
+        f_prev = lat.populations()
+        lat.step(1)
+        r = lat.reduce()
+        f = lat.populations()
+    assert r["mass"] == pytest.approx(f.sum(), rel=1e-13)
+    assert np.allclose(r["momentum"], f.T.reshape(q, -1).sum(1) @ e, rtol=0, atol=1e-12)
+    want = np.linalg.norm(f - f_prev) / np.linalg.norm(f)
+    assert r["residual"] == pytest.approx(want, rel=1e-10)
+
+
+def _run(q, dims, nd, storage, steps, **kw):
+    with Lattice(q, *dims, nd, storage=storage, **kw) as lat:
+        lat.step(steps)
+        m = lat.macros()
+        red = lat.reduce()
+    return m, red
+
+
+def _velocity(m, fluid):
+    return np.stack([m["ux"][fluid], m["uy"][fluid], m["uz"][fluid]])
+
+
+@pytest.mark.parametrize("q", [15, 19, 27])
+def test_fp32_and_fp16_velocity_error_vs_fp64(gpu_lib, q):
+    dims = (24, 24, 64)
+    nd = geometry.channel(*dims, obstacles=[geometry.sphere(11, 12, 20, 5)])
+    fluid = nd == 0
+    kw = dict(omega=1.0 / 0.6, u_lbm=0.05, rho_lbm=1.0)
+    m64, r64 = _run(q, dims, nd, "f64", 600, **kw)
+    m32, r32 = _run(q, dims, nd, "f32", 600, **kw)
+    m16, r16 = _run(q, dims, nd, "f16", 600, **kw)
+    u64 = _velocity(m64, fluid)
+    e32 = l2_rel(_velocity(m32, fluid), u64)
+    e16 = l2_rel(_velocity(m16, fluid), u64)
+    d32 = abs(r32["mass"] - r64["mass"]) / r64["mass"]
+    d16 = abs(r16["mass"] - r64["mass"]) / r64["mass"]
+    print(f"q={q} L2 fp32 {e32:.3e} fp16 {e16:.3e}; mass drift fp32 {d32:.3e} fp16 {d16:.3e}")
+    assert e32 <= 1e-5
+    assert e16 <= 1e-3
+
+
+def test_fp32_matches_oracle_fp64_on_config_channel(gpu_lib):
+    """The 24x24x96 D3Q15 channel: FP32 storage within L2 1e-5 of the
+    reference's FP64 result after 2000 steps."""
+    case = [c for c in CASES if c.name.startswith("channel_d3q15")][0]
+    fluid = case.nd == 0
+    ref = np.stack([case.out[k][fluid] for k in ("ux", "uy", "uz")])
+    with Lattice(case.q, *case.dims, case.nd, storage="f32", **case.kwargs()) as lat:
+        lat.step(case.params["numTs"])
+        m = lat.macros()
+    got = np.stack([m[k][fluid] for k in ("ux", "uy", "uz")])
+    assert l2_rel(got, ref) <= 1e-5
+
+
+def test_translation_invariance_periodic_box(gpu_lib):
+    """Shifting state and geometry periodically commutes with the step
+    (size-independent; exact in every storage)."""
+    nx, ny, nz, q = 32, 24, 40, 19
+    nd = geometry.periodic_box(nx, ny, nz, obstacles=[geometry.sphere(9, 7, 11, 4.5)])
+    rng = np.random.default_rng(2)
+    o = Oracle(q, 1, 1, 1, [0], omega=1.0)
+    f0 = _random_state(rng, o.weights(), nx * ny * nz)
+    shift = (5, -3, 7)
+
+    def roll(a):
+        a = a.reshape(nz, ny, nx, *a.shape[1:])
+        return np.roll(a, (shift[2], shift[1], shift[0]), axis=(0, 1, 2)).reshape(nx * ny * nz, *a.shape[3:])
+
+    for storage in ("f64", "f32", "f16"):
+        outs = []
+        for rolled in (False, True):
+            with Lattice(q, nx, ny, nz, roll(nd) if rolled else nd, storage=storage, omega=1.7,
+                         cs=0.1) as lat:
+                lat.set_populations(roll(f0) if rolled else f0)
+                lat.step(5)
+                outs.append(lat.populations())
+        assert np.array_equal(roll(outs[0]), outs[1]), storage

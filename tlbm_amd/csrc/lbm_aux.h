@@ -1,0 +1,224 @@
+// lbm_aux.h -- kernels around the time step: initial and restart state,
+// population import/export, the momentum-exchange force and the FP64
+// conservation / residual reductions.  All of them address the pull state
+// B (post-collision, not yet streamed; see lbm_kernels.h): the populations
+// the next step reads are A[g,s] = B[s][g - e_s].
+#pragma once
+
+#include "lbm_kernels.h"
+
+namespace tlbm {
+
+// Periodic coordinate shift by d (any sign, |d| <= 2n).
+__device__ inline uint32_t wrap(int v, int n) {
+    v %= n;
+    return uint32_t(v < 0 ? v + n : v);
+}
+
+struct Coord {
+    int x, y, z;
+    __device__ inline Coord(uint32_t g, const StepArgs &a) {
+        z = int(g / a.nxny);
+        const uint32_t r = g - uint32_t(z) * a.nxny;
+        y = int(r / a.nx);
+        x = int(r - uint32_t(y) * a.nx);
+    }
+    __device__ inline uint32_t shifted(int dx, int dy, int dz, const StepArgs &a) const {
+        return wrap(x + dx, int(a.nx)) + wrap(y + dy, int(a.ny)) * a.nx +
+               wrap(z + dz, int(a.nz)) * a.nxny;
+    }
+};
+
+// f_eq of one direction, LatticeStructure.hpp:92-103 in the reference's
+// operation order.
+template <class L, class T>
+__device__ inline T feq_one(int s, T w, T rho, T ux, T uy, T uz) {
+    const T usq = ux * ux + uy * uy + uz * uz;
+    const T cu = (s == 0) ? T(0) : T(3) * edot<T>(L::ex[s], L::ey[s], L::ez[s], ux, uy, uz);
+    return w * rho * (T(1) + cu + T(0.5) * (cu * cu) - T(1.5) * usq);
+}
+
+// B[s][h] = f_eq_s(rho, u at node h + e_s): the pull state whose next step
+// reads f_eq(rho, u) at every node (restart, TLBM_Partition.cpp:527-532).
+// mac holds ux, uy, uz, rho planes of N doubles.  FP64 storage evaluates
+// f_eq as the reference does; FP32/FP16 store f_eq - w, evaluated in double
+// as w (delta + rho (cu + cu^2/2 - 3u^2/2)).
+template <class L, int ST>
+__global__ void __launch_bounds__(kBlock)
+k_init_macros(typename StorageT<ST>::st *__restrict__ fout, const double *__restrict__ mac,
+              StepArgs a) {
+    using S = StorageT<ST>;
+    using T = typename S::T;
+    const uint32_t h = blockIdx.x * kBlock + threadIdx.x;
+    if (h >= a.N) return;
+    const Coord c(h, a);
+#pragma unroll
+    for (int s = 0; s < L::Q; ++s) {
+        const uint32_t src = c.shifted(L::ex[s], L::ey[s], L::ez[s], a);
+        const double ux = mac[src], uy = mac[size_t(a.N) + src], uz = mac[2 * size_t(a.N) + src],
+                     rho = mac[3 * size_t(a.N) + src];
+        T v;
+        if constexpr (S::kShift) {
+            const double w = double(a.wf[s]);
+            const double usq = ux * ux + uy * uy + uz * uz;
+            const double cu = (s == 0) ? 0.0 : 3.0 * edot<double>(L::ex[s], L::ey[s], L::ez[s], ux, uy, uz);
+            v = T(w * ((rho - 1.0) + rho * (cu + 0.5 * (cu * cu) - 1.5 * usq)));
+        } else {
+            v = feq_one<L, T>(s, S::weight(s, a), T(rho), T(ux), T(uy), T(uz));
+        }
+        S::store(fout + size_t(s) * a.P + h, s, v, a);
+    }
+}
+
+// Uniform rest state f = w * rho (initialize_data_arrays,
+// TLBM_Partition.cpp:458-465); it is its own stream, so B = A.
+template <class L, int ST>
+__global__ void __launch_bounds__(kBlock)
+k_init_rest(typename StorageT<ST>::st *__restrict__ fout, StepArgs a) {
+    using S = StorageT<ST>;
+    using T = typename S::T;
+    const uint32_t h = blockIdx.x * kBlock + threadIdx.x;
+    if (h >= a.N) return;
+#pragma unroll
+    for (int s = 0; s < L::Q; ++s) {
+        T v;
+        if constexpr (S::kShift)
+            v = T(double(a.wf[s]) * a.rho_bc - double(a.wf[s]));
+        else
+            v = S::weight(s, a) * T(a.rho_bc);
+        S::store(fout + size_t(s) * a.P + h, s, v, a);
+    }
+}
+
+// f_aos[g*Q + s] = A[g,s] = B[s][g - e_s]
+template <class L, int ST>
+__global__ void __launch_bounds__(kBlock)
+k_get_populations(const typename StorageT<ST>::st *__restrict__ fin, double *__restrict__ f_aos,
+                  StepArgs a) {
+    using S = StorageT<ST>;
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= a.N) return;
+    const Coord c(g, a);
+#pragma unroll
+    for (int s = 0; s < L::Q; ++s) {
+        const uint32_t src = c.shifted(-L::ex[s], -L::ey[s], -L::ez[s], a);
+        f_aos[size_t(g) * L::Q + s] = S::absolute(S::load(fin + size_t(s) * a.P + src, s, a), s, a);
+    }
+}
+
+// B[s][h] = f_aos[(h + e_s)*Q + s]
+template <class L, int ST>
+__global__ void __launch_bounds__(kBlock)
+k_set_populations(typename StorageT<ST>::st *__restrict__ fout, const double *__restrict__ f_aos,
+                  StepArgs a) {
+    using S = StorageT<ST>;
+    const uint32_t h = blockIdx.x * kBlock + threadIdx.x;
+    if (h >= a.N) return;
+    const Coord c(h, a);
+#pragma unroll
+    for (int s = 0; s < L::Q; ++s) {
+        const uint32_t src = c.shifted(L::ex[s], L::ey[s], L::ez[s], a);
+        S::store(fout + size_t(s) * a.P + h, s, S::from_absolute(f_aos[size_t(src) * L::Q + s], s, a), a);
+    }
+}
+
+// calc_force (TLBM_Partition.cpp:174-208) over the map of
+// make_force_calc_map (143-172): for a solid node g and each direction spd
+// (ascending) whose neighbour g+e_spd is a fluid (type 0) node,
+//   f1 = A[g, spd], f2 = A[g+e_spd, bb(spd)],  F += e_bb(spd) * (f1 + f2).
+// In the pull state f1 = B[spd][g - e_spd] and f2 = B[bb][g + 2 e_spd].
+// F holds Fx, Fy, Fz planes of N doubles.
+template <class L, int ST>
+__global__ void __launch_bounds__(kBlock)
+k_forces(const typename StorageT<ST>::st *__restrict__ fin, const uint8_t *__restrict__ nd_type,
+         double *__restrict__ F, StepArgs a) {
+    using S = StorageT<ST>;
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= a.N) return;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (nd_type[g] == 1) {
+        const Coord c(g, a);
+#pragma unroll
+        for (int s = 0; s < L::Q; ++s) {
+            const uint32_t ngb = c.shifted(L::ex[s], L::ey[s], L::ez[s], a);
+            if (nd_type[ngb] != 0) continue;
+            const int b = L::bb[s];
+            const double f1 = S::absolute(
+                S::load(fin + size_t(s) * a.P + c.shifted(-L::ex[s], -L::ey[s], -L::ez[s], a), s, a), s, a);
+            const double f2 = S::absolute(
+                S::load(fin + size_t(b) * a.P + c.shifted(2 * L::ex[s], 2 * L::ey[s], 2 * L::ez[s], a), b, a), b, a);
+            const double sum = f1 + f2;
+            if (L::ex[b] > 0) fx += sum; else if (L::ex[b] < 0) fx -= sum;
+            if (L::ey[b] > 0) fy += sum; else if (L::ey[b] < 0) fy -= sum;
+            if (L::ez[b] > 0) fz += sum; else if (L::ez[b] < 0) fz -= sum;
+        }
+    }
+    F[g] = fx;
+    F[size_t(a.N) + g] = fy;
+    F[2 * size_t(a.N) + g] = fz;
+}
+
+// Per-workgroup FP64 partial sums over the state: sum f, sum f e_x,
+// sum f e_y, sum f e_z, sum (B_n - B_{n-1})^2, sum B_n^2 (kRed values).
+
+template <class L, int ST>
+__global__ void __launch_bounds__(kBlock)
+k_reduce_partial(const typename StorageT<ST>::st *__restrict__ cur,
+                 const typename StorageT<ST>::st *__restrict__ prev, double *__restrict__ partial,
+                 StepArgs a) {
+    using S = StorageT<ST>;
+    double acc[kRed] = {0, 0, 0, 0, 0, 0};
+    for (uint32_t g = blockIdx.x * kBlock + threadIdx.x; g < a.N; g += gridDim.x * kBlock) {
+#pragma unroll
+        for (int s = 0; s < L::Q; ++s) {
+            const double v = S::absolute(S::load(cur + size_t(s) * a.P + g, s, a), s, a);
+            acc[0] += v;
+            acc[1] += L::ex[s] * v;
+            acc[2] += L::ey[s] * v;
+            acc[3] += L::ez[s] * v;
+            acc[5] += v * v;
+            if (prev) {
+                const double d = v - S::absolute(S::load(prev + size_t(s) * a.P + g, s, a), s, a);
+                acc[4] += d * d;
+            }
+        }
+    }
+    __shared__ double sh[kRed][kBlock];
+#pragma unroll
+    for (int k = 0; k < kRed; ++k) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) {
+#pragma unroll
+            for (int k = 0; k < kRed; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kRed) partial[size_t(blockIdx.x) * kRed + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// One workgroup folds the partials in a fixed order (deterministic).
+// Templated on the storage only so each kernel file owns its copy.
+template <int ST>
+__global__ void __launch_bounds__(kBlock)
+k_reduce_final(const double *__restrict__ partial, int nparts, double *__restrict__ out) {
+    __shared__ double sh[kRed][kBlock];
+    double acc[kRed] = {0, 0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < nparts; i += kBlock) {
+#pragma unroll
+        for (int k = 0; k < kRed; ++k) acc[k] += partial[size_t(i) * kRed + k];
+    }
+#pragma unroll
+    for (int k = 0; k < kRed; ++k) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) {
+#pragma unroll
+            for (int k = 0; k < kRed; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < kRed) out[threadIdx.x] = sh[threadIdx.x][0];
+}
+
+}  // namespace tlbm

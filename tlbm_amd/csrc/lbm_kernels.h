@@ -1,0 +1,344 @@
+// lbm_kernels.h -- fused pull stream-collide kernel for gfx950 and the
+// per-node code it shares with the moment (dump) and type-5 helper passes.
+//
+// Layout in HBM: structure of arrays, one plane per direction s with a
+// stride of P >= N values: f[s*P + g], g = x + nx*y + nx*ny*z
+// (TLBM_Partition.cpp:730-737).  P is padded past N so that the 2*Q
+// streams of a step do not start at addresses a power of two apart.
+// Two state buffers ping-pong.  A state buffer holds post-collision,
+// not-yet-streamed populations B_n; step n+1 gathers
+// A[g,s] = B_n[g - e_s (periodic), s], which is exactly what the reference's
+// push stream (TLBM_Partition.cpp:826-837) would have delivered to g,
+// collides at g and writes B_{n+1}[s][g] aligned.  The periodic wrap in x, y
+// and z (TLBM_Partition.cpp:691-712) is index arithmetic in the gather, so
+// no halo or adjacency table exists.
+//
+// Per-node order and arithmetic follow TLBM_Partition.cpp:742-809 and
+// include/LatticeStructure.hpp:69-177.  In double precision every operation
+// is the reference's, in the same order (the FP64 kernels are compiled
+// without floating-point contraction), so the FP64 path reproduces the
+// reference's rho and u.  Terms that multiply a population or velocity by a
+// zero velocity component are left out: they add +-0, which leaves every
+// nonzero sum unchanged.
+//
+// FP32 and FP16 storage keep d_s = f_s - w_s (FP16 scaled by 1/w_s) and the
+// FP32 arithmetic works on d and on delta = rho - 1 throughout: the same
+// formulas rewritten exactly (w is symmetric under bounce-back, sum w e = 0),
+// without the cancellation of O(1) values that costs FP32 its accuracy.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lattice.h"
+
+namespace tlbm {
+
+enum Storage : int { ST_F64 = 0, ST_F32 = 1, ST_F16 = 2 };
+enum Mode : int { MODE_STEP = 0, MODE_MACROS = 1, MODE_HEAD = 2 };
+
+constexpr int kBlock = 256;  // 4 wave64 per workgroup
+constexpr int kRed = 6;      // values per FP64 reduction partial
+
+// Launch arguments, passed by value (kernarg segment, scalar registers).
+struct StepArgs {
+    uint32_t nx, ny, nz, nxny, N;
+    uint32_t P;        // plane stride in values (>= N)
+    int dynamics;      // 1: BGK relaxation, other values: none (TLBM_Partition.cpp:795-807)
+    int smag;          // cs > 0
+    double cs, omega, nu, nu2, u_bc, rho_bc, inv_1mu, inv_rho_bc;
+    double w[27];      // weights (the reference's float-literal values unless exact)
+    float wf[27];      // the same weights as float (FP32/FP16 compute)
+    float h_scale[27];     // FP16 storage: h = d * h_scale (= d / w)
+    float h_unscale[27];   // FP16 storage: d = h * h_unscale (= h * w)
+    // deviation form (FP32/FP16), each computed in double on the host
+    float dw;          // sum_s w_s - 1
+    float c_in;        // 2*sum_{e_z=-1} w + sum_{e_z=0} w - (1 - u_bc)
+    float c_out;       // 2*sum_{e_z=+1} w + sum_{e_z=0} w - rho_bc
+    float drho_bc;     // rho_bc - 1
+};
+
+// Raw value <-> compute value.  kShift: the compute value is d = f - w.
+template <int ST> struct StorageT;
+template <> struct StorageT<ST_F64> {
+    using st = double;
+    using T = double;
+    static constexpr bool kShift = false;
+    __device__ static inline T load(const st *p, int, const StepArgs &) { return *p; }
+    __device__ static inline void store(st *p, int, T v, const StepArgs &) {
+        __builtin_nontemporal_store(v, p);
+    }
+    __device__ static inline T weight(int s, const StepArgs &a) { return a.w[s]; }
+    __device__ static inline double absolute(T v, int, const StepArgs &) { return v; }
+    __device__ static inline T from_absolute(double f, int, const StepArgs &) { return f; }
+};
+template <> struct StorageT<ST_F32> {
+    using st = float;
+    using T = float;
+    static constexpr bool kShift = true;
+    __device__ static inline T load(const st *p, int, const StepArgs &) { return *p; }
+    __device__ static inline void store(st *p, int, T v, const StepArgs &) {
+        __builtin_nontemporal_store(v, p);
+    }
+    __device__ static inline T weight(int s, const StepArgs &a) { return a.wf[s]; }
+    __device__ static inline double absolute(T v, int s, const StepArgs &a) {
+        return double(a.wf[s]) + double(v);
+    }
+    __device__ static inline T from_absolute(double f, int s, const StepArgs &a) {
+        return T(f - double(a.wf[s]));
+    }
+};
+template <> struct StorageT<ST_F16> {
+    using st = _Float16;
+    using T = float;
+    static constexpr bool kShift = true;
+    __device__ static inline T load(const st *p, int s, const StepArgs &a) {
+        return float(*p) * a.h_unscale[s];
+    }
+    __device__ static inline void store(st *p, int s, T v, const StepArgs &a) {
+        __builtin_nontemporal_store(_Float16(v * a.h_scale[s]), p);
+    }
+    __device__ static inline T weight(int s, const StepArgs &a) { return a.wf[s]; }
+    __device__ static inline double absolute(T v, int s, const StepArgs &a) {
+        return double(a.wf[s]) + double(v);
+    }
+    __device__ static inline T from_absolute(double f, int s, const StepArgs &a) {
+        return T(f - double(a.wf[s]));
+    }
+};
+
+// Workgroup -> node block.  The dispatcher deals workgroups round-robin to
+// the 8 XCDs; this hands XCD k one contiguous range of node blocks so the
+// cache lines two neighbouring blocks share stay in one L2.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t per = nb >> 3, rem = nb & 7u;
+    const uint32_t k = b & 7u, slot = b >> 3;
+    return k * per + (k < rem ? k : rem) + slot;
+}
+
+// Pull source offsets of node (x,y,z): index e+1 holds the coordinate of
+// the node at -e (periodic), scaled to a global-index offset.
+struct Gather {
+    uint32_t xs[3], ys[3], zs[3];
+    __device__ inline void init(uint32_t x, uint32_t y, uint32_t z, const StepArgs &a) {
+        xs[0] = (x + 1 == a.nx) ? 0 : x + 1;  // e_x = -1 comes from x+1
+        xs[1] = x;
+        xs[2] = (x == 0) ? a.nx - 1 : x - 1;  // e_x = +1 comes from x-1
+        ys[0] = ((y + 1 == a.ny) ? 0 : y + 1) * a.nx;
+        ys[1] = y * a.nx;
+        ys[2] = ((y == 0) ? a.ny - 1 : y - 1) * a.nx;
+        zs[0] = ((z + 1 == a.nz) ? 0 : z + 1) * a.nxny;
+        zs[1] = z * a.nxny;
+        zs[2] = ((z == 0) ? a.nz - 1 : z - 1) * a.nxny;
+    }
+    __device__ inline uint32_t src(int ex, int ey, int ez) const {
+        return xs[ex + 1] + ys[ey + 1] + zs[ez + 1];
+    }
+};
+
+// sum_k e_k u_k over the nonzero components, in x, y, z order.
+template <class T>
+__device__ inline T edot(int ex, int ey, int ez, T ux, T uy, T uz) {
+    T d = T(-0.0);
+    if (ex > 0) d += ux; else if (ex < 0) d -= ux;
+    if (ey > 0) d += uy; else if (ey < 0) d -= uy;
+    if (ez > 0) d += uz; else if (ez < 0) d -= uz;
+    return d;
+}
+
+// The reference's node update on registers.  f[] holds the gathered
+// pre-collision populations on entry (deviations d = f - w when SHIFT) and
+// the post-collision ones on exit; rho/ux/uy/uz end as the values
+// write_data() would dump for this step.
+template <class L, class T, bool SHIFT, bool COLLIDE>
+__device__ inline void node_update(T (&f)[L::Q], int t, T &rho, T &ux, T &uy, T &uz,
+                                   const StepArgs &a, const T *__restrict__ uy_head,
+                                   const T (&w)[L::Q]) {
+    constexpr int Q = L::Q;
+    T dlt = T(0);  // rho - 1 (SHIFT only)
+    // compute_macroscopic_data, LatticeStructure.hpp:69-90
+    {
+        T r = T(-0.0), mx = T(-0.0), my = T(-0.0), mz = T(-0.0);
+#pragma unroll
+        for (int s = 0; s < Q; ++s) {
+            r += f[s];
+            if (L::ex[s] > 0) mx += f[s]; else if (L::ex[s] < 0) mx -= f[s];
+            if (L::ey[s] > 0) my += f[s]; else if (L::ey[s] < 0) my -= f[s];
+            if (L::ez[s] > 0) mz += f[s]; else if (L::ez[s] < 0) mz -= f[s];
+        }
+        if constexpr (SHIFT) {
+            dlt = r + T(a.dw);
+            rho = T(1) + dlt;
+            const T inv = T(1) / rho;
+            ux = mx * inv;
+            uy = my * inv;
+            uz = mz * inv;
+        } else {
+            rho = r;
+            ux = mx / r;
+            uy = my / r;
+            uz = mz / r;
+        }
+    }
+    if (t == 1) {
+        // solid node: u = 0, whole-vector bounce-back, no relaxation
+        // (TLBM_Partition.cpp:750-756, D3Q19LatticeStructure.hpp:65-76)
+        ux = T(0); uy = T(0); uz = T(0);
+        if (COLLIDE) {
+            T g[Q];
+#pragma unroll
+            for (int s = 0; s < Q; ++s) g[s] = f[L::bb[s]];
+#pragma unroll
+            for (int s = 0; s < Q; ++s) f[s] = g[s];
+        }
+        return;
+    }
+    if (t == 2) {
+        // set_inletW_bc_macro (D3Q19LatticeStructure.hpp:108-132)
+        T s_in = T(-0.0), sz = T(-0.0);
+#pragma unroll
+        for (int i = 0; i < L::n_unk; ++i) s_in += f[L::in_list[i]];
+#pragma unroll
+        for (int i = 0; i < L::n_zero; ++i) sz += f[L::zero_list[i]];
+        ux = T(0); uy = T(0); uz = T(a.u_bc);
+        if constexpr (SHIFT) {
+            dlt = T(a.inv_1mu) * (T(a.c_in) + (T(2) * s_in + sz));
+            rho = T(1) + dlt;
+        } else {
+            rho = T(a.inv_1mu) * (T(2) * s_in + sz);
+        }
+    } else if (t == 3) {
+        // set_outletE_bc_macro (D3Q19LatticeStructure.hpp:135-156); ux, uy kept
+        T s_out = T(-0.0), sz = T(-0.0);
+#pragma unroll
+        for (int i = 0; i < L::n_unk; ++i) s_out += f[L::out_list[i]];
+#pragma unroll
+        for (int i = 0; i < L::n_zero; ++i) sz += f[L::zero_list[i]];
+        rho = T(a.rho_bc);
+        if constexpr (SHIFT) {
+            dlt = T(a.drho_bc);
+            uz = T(a.inv_rho_bc) * (T(a.c_out) + (T(2) * s_out + sz));
+        } else {
+            uz = T(-1) + T(a.inv_rho_bc) * (T(2) * s_out + sz);
+        }
+    }
+    if (!COLLIDE) return;
+    if (t == 5) {
+        // set_uz_bc (LatticeStructure.hpp:166-177); the y term reads uy of
+        // node s of this step, as the reference's uy[spd] does
+        const T ubc = T(a.u_bc);
+#pragma unroll
+        for (int s = 1; s < Q; ++s) {
+            T br = T(-0.0);
+            if (L::ez[s] > 0) br += (ubc - uz); else if (L::ez[s] < 0) br -= (ubc - uz);
+            if (L::ex[s] > 0) br -= ux; else if (L::ex[s] < 0) br += ux;
+            if (L::ey[s] > 0) br -= uy_head[s]; else if (L::ey[s] < 0) br += uy_head[s];
+            f[s] = f[s] + T(3) * rho * w[s] * br;
+        }
+    }
+    // compute_equilibrium, LatticeStructure.hpp:92-103
+    // (SHIFT: f_eq - w = w (delta + rho (cu + cu^2/2 - 3u^2/2)))
+    T feq[Q];
+    {
+        const T usq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+        for (int s = 0; s < Q; ++s) {
+            const T cu = (s == 0) ? T(0) : T(3) * edot<T>(L::ex[s], L::ey[s], L::ez[s], ux, uy, uz);
+            if constexpr (SHIFT)
+                feq[s] = w[s] * (dlt + rho * (cu + T(0.5) * (cu * cu) - T(1.5) * usq));
+            else
+                feq[s] = w[s] * rho * (T(1) + cu + T(0.5) * (cu * cu) - T(1.5) * usq);
+        }
+    }
+    if (t == 2) {
+#pragma unroll
+        for (int i = 0; i < L::n_unk; ++i)
+            f[L::in_unk[i]] = feq[L::in_unk[i]] + f[L::in_par[i]] - feq[L::in_par[i]];
+    } else if (t == 3) {
+#pragma unroll
+        for (int i = 0; i < L::n_unk; ++i)
+            f[L::out_unk[i]] = feq[L::out_unk[i]] + f[L::out_par[i]] - feq[L::out_par[i]];
+    }
+    T om = T(a.omega);
+    if (a.smag) {
+        // compute_strain_tensor + apply_turbulence_model,
+        // LatticeStructure.hpp:105-122, 140-153
+        T sxx = T(-0.0), syy = T(-0.0), szz = T(-0.0), sxy = T(-0.0), sxz = T(-0.0),
+          syz = T(-0.0);
+#pragma unroll
+        for (int s = 0; s < Q; ++s) {
+            const T fn = f[s] - feq[s];
+            if (L::ex[s] != 0) sxx += fn;
+            if (L::ey[s] != 0) syy += fn;
+            if (L::ez[s] != 0) szz += fn;
+            if (L::ex[s] * L::ey[s] > 0) sxy += fn; else if (L::ex[s] * L::ey[s] < 0) sxy -= fn;
+            if (L::ex[s] * L::ez[s] > 0) sxz += fn; else if (L::ex[s] * L::ez[s] < 0) sxz -= fn;
+            if (L::ey[s] * L::ez[s] > 0) syz += fn; else if (L::ey[s] * L::ez[s] < 0) syz -= fn;
+        }
+        T P = sqrt(sxx * sxx + syy * syy + szz * szz + T(2) * (sxy * sxy + sxz * sxz + syz * syz));
+        P *= T(a.cs);
+        P = sqrt(P + T(a.nu2)) - T(a.nu);
+        const T nu_e = P / T(6);
+        om = T(1) / (T(3) * (T(a.nu) + nu_e) + T(0.5));
+    }
+    if (a.dynamics == 1) {
+#pragma unroll
+        for (int s = 0; s < Q; ++s) f[s] = f[s] - om * (f[s] - feq[s]);
+    }
+}
+
+// One lattice update per thread.  MODE_STEP streams and collides
+// fin -> fout; MODE_MACROS recomputes the moments of the step that read fin
+// (ux, uy, uz, rho planes of N values in `macro`); MODE_HEAD writes uy of
+// nodes 0..Q-1 (the values the type-5 rule reads) to `macro`.
+template <class L, int ST, int MODE>
+__global__ void __launch_bounds__(kBlock)
+k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::st *__restrict__ fout,
+      const uint8_t *__restrict__ nd_type, typename StorageT<ST>::T *__restrict__ macro,
+      const typename StorageT<ST>::T *__restrict__ uy_head, StepArgs a) {
+    using S = StorageT<ST>;
+    using T = typename S::T;
+    constexpr int Q = L::Q;
+    uint32_t g;
+    if (MODE == MODE_HEAD) {
+        g = threadIdx.x;
+        if (g >= (uint32_t)Q || g >= a.N) return;
+    } else {
+        g = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+        if (g >= a.N) return;
+    }
+    const uint32_t z = g / a.nxny;
+    const uint32_t r = g - z * a.nxny;
+    const uint32_t y = r / a.nx;
+    const uint32_t x = r - y * a.nx;
+    Gather gat;
+    gat.init(x, y, z, a);
+    T f[Q];
+#pragma unroll
+    for (int s = 0; s < Q; ++s) {
+        const size_t plane = size_t(s) * a.P;
+        f[s] = S::load(fin + plane + gat.src(L::ex[s], L::ey[s], L::ez[s]), s, a);
+    }
+    T w[Q];
+#pragma unroll
+    for (int s = 0; s < Q; ++s) w[s] = S::weight(s, a);
+    const int t = nd_type[g];
+    T rho, ux, uy, uz;
+    if (MODE == MODE_STEP) {
+        node_update<L, T, S::kShift, true>(f, t, rho, ux, uy, uz, a, uy_head, w);
+#pragma unroll
+        for (int s = 0; s < Q; ++s) S::store(fout + size_t(s) * a.P + g, s, f[s], a);
+    } else {
+        node_update<L, T, S::kShift, false>(f, t, rho, ux, uy, uz, a, uy_head, w);
+        if (MODE == MODE_MACROS) {
+            macro[g] = ux;
+            macro[size_t(a.N) + g] = uy;
+            macro[2 * size_t(a.N) + g] = uz;
+            macro[3 * size_t(a.N) + g] = rho;
+        } else {
+            macro[g] = uy;
+        }
+    }
+}
+
+}  // namespace tlbm

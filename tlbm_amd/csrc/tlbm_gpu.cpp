@@ -1,0 +1,412 @@
+// tlbm_gpu.cpp -- the C-ABI of include/tlbm_gpu.h over the HIP kernels.
+#include "tlbm_gpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "launch.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+#define TLBM_HIP(call)                                                                       \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(TLBM_EHIP, std::string(#call) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+template <class L>
+void fill_weights(tlbm::StepArgs &a, bool exact) {
+    for (int s = 0; s < L::Q; ++s) {
+        a.w[s] = exact ? L::w_exact[s] : L::w_ref[s];
+        a.wf[s] = float(a.w[s]);
+        // FP16 storage keeps (f - w)/w: a relative deviation, O(0.1) at Ma ~ 0.1
+        a.h_scale[s] = float(1.0 / double(a.wf[s]));
+        a.h_unscale[s] = a.wf[s];
+    }
+    // constants of the deviation form, from the float weights the FP32
+    // arithmetic uses, evaluated in double
+    double sum = 0, s_in = 0, s_out = 0, s_zero = 0;
+    for (int s = 0; s < L::Q; ++s) sum += double(a.wf[s]);
+    for (int i = 0; i < L::n_unk; ++i) {
+        s_in += double(a.wf[L::in_list[i]]);
+        s_out += double(a.wf[L::out_list[i]]);
+    }
+    for (int i = 0; i < L::n_zero; ++i) s_zero += double(a.wf[L::zero_list[i]]);
+    a.dw = float(sum - 1.0);
+    a.c_in = float(2.0 * s_in + s_zero - (1.0 - a.u_bc));
+    a.c_out = float(2.0 * s_out + s_zero - a.rho_bc);
+    a.drho_bc = float(a.rho_bc - 1.0);
+}
+
+// Plane stride: N rounded up to whole 256-byte lines plus an odd number of
+// lines, so the 2*Q streams of a step never start a power of two apart.
+uint32_t plane_stride(uint32_t N, size_t value_bytes) {
+    const uint32_t line = uint32_t(256 / value_bytes);
+    return ((N + line - 1) / line) * line + 3 * line;
+}
+
+int lattice_q(int lattice) {
+    return (lattice == 15 || lattice == 19 || lattice == 27) ? lattice : 0;
+}
+
+const tlbm::LaunchOps &ops_for(int storage) {
+    switch (storage) {
+    case TLBM_STORAGE_F64: return tlbm::ops_f64();
+    case TLBM_STORAGE_F32: return tlbm::ops_f32();
+    default: return tlbm::ops_f16();
+    }
+}
+
+size_t storage_bytes(int storage) {
+    return storage == TLBM_STORAGE_F64 ? 8 : storage == TLBM_STORAGE_F32 ? 4 : 2;
+}
+
+}  // namespace
+
+struct tlbm_gpu {
+    tlbm_params p{};
+    int Q = 0;
+    uint32_t N = 0;
+    tlbm::StepArgs args{};
+    const tlbm::LaunchOps *ops = nullptr;
+    void *f[2] = {nullptr, nullptr};
+    int cur = 0;              // f[cur] holds the state the next step reads from
+    uint8_t *nd = nullptr;
+    void *macro = nullptr;    // 4*N compute-type values (dumps)
+    void *uy_head = nullptr;  // Q compute-type values (type-5 rule)
+    double *scratch = nullptr;  // staging for forces / populations / macros (lazy)
+    size_t scratch_bytes = 0;
+    double *partial = nullptr;  // reduction partials
+    int nparts = 0;
+    bool has_lid = false;
+    long long steps_since_set = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint64_t bytes = 0;
+};
+
+namespace {
+
+int ensure_scratch(tlbm_gpu *h, size_t bytes) {
+    if (h->scratch_bytes >= bytes) return TLBM_OK;
+    if (h->scratch) {
+        TLBM_HIP(hipFree(h->scratch));
+        h->bytes -= h->scratch_bytes;
+        h->scratch = nullptr;
+        h->scratch_bytes = 0;
+    }
+    TLBM_HIP(hipMalloc(&h->scratch, bytes));
+    h->scratch_bytes = bytes;
+    h->bytes += bytes;
+    return TLBM_OK;
+}
+
+int check(const tlbm_gpu *h) {
+    if (!h) return fail(TLBM_EINVAL, "null handle");
+    return TLBM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *tlbm_gpu_last_error(void) { return g_err.c_str(); }
+
+int tlbm_gpu_create(const tlbm_params *p, const uint8_t *nd_type, tlbm_gpu **out) {
+    g_err.clear();
+    if (!p || !nd_type || !out) return fail(TLBM_EINVAL, "null argument");
+    *out = nullptr;
+    const int Q = lattice_q(p->lattice);
+    if (!Q) return fail(TLBM_EINVAL, "lattice must be 15, 19 or 27 (D3Q15/D3Q19/D3Q27)");
+    if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0)
+        return fail(TLBM_EINVAL, "nx, ny, nz must be positive");
+    const uint64_t N64 = uint64_t(p->nx) * uint64_t(p->ny) * uint64_t(p->nz);
+    if (N64 >= (uint64_t(1) << 31) - 4096)
+        return fail(TLBM_EINVAL, "lattice has 2^31 - 4096 nodes or more");
+    if (p->storage < TLBM_STORAGE_F64 || p->storage > TLBM_STORAGE_F16)
+        return fail(TLBM_EINVAL, "unknown storage");
+    if (p->reserved != 0) return fail(TLBM_EINVAL, "reserved field must be 0");
+    if (!(p->omega > 0.0) || !std::isfinite(p->omega)) return fail(TLBM_EINVAL, "omega must be > 0");
+    if (!(1.0 - p->u_lbm != 0.0)) return fail(TLBM_EINVAL, "u_lbm must differ from 1");
+    if (!(p->rho_lbm != 0.0)) return fail(TLBM_EINVAL, "rho_lbm must be nonzero");
+    const uint32_t N = uint32_t(N64);
+    bool has_lid = false;
+    for (uint32_t g = 0; g < N; ++g) {
+        const uint8_t t = nd_type[g];
+        if (!(t == 0 || t == 1 || t == 2 || t == 3 || t == 5)) {
+            char buf[128];
+            std::snprintf(buf, sizeof buf, "node %u has type %u; valid types are 0,1,2,3,5", g,
+                          unsigned(t));
+            return fail(TLBM_EINVAL, buf);
+        }
+        if (t == 5) {
+            has_lid = true;
+            if (g + 1 < uint32_t(Q)) {
+                char buf[160];
+                std::snprintf(buf, sizeof buf,
+                              "type-5 node %u precedes node %d, whose uy its rule reads", g, Q - 1);
+                return fail(TLBM_EINVAL, buf);
+            }
+        }
+    }
+    tlbm_gpu *h = new (std::nothrow) tlbm_gpu;
+    if (!h) return fail(TLBM_ENOMEM, "host allocation failed");
+    h->p = *p;
+    h->Q = Q;
+    h->N = N;
+    h->has_lid = has_lid;
+    h->ops = &ops_for(p->storage);
+    tlbm::StepArgs &a = h->args;
+    a.nx = uint32_t(p->nx);
+    a.ny = uint32_t(p->ny);
+    a.nz = uint32_t(p->nz);
+    a.nxny = a.nx * a.ny;
+    a.N = N;
+    a.dynamics = p->dynamics;
+    a.smag = p->cs > 0.0;
+    a.cs = p->cs;
+    a.omega = p->omega;
+    a.nu = ((1. / p->omega) - 0.5) / 3.0;  // LatticeStructure.hpp:145
+    a.nu2 = a.nu * a.nu;
+    a.u_bc = p->u_lbm;
+    a.rho_bc = p->rho_lbm;
+    a.inv_1mu = 1. / (1. - p->u_lbm);
+    a.inv_rho_bc = 1. / p->rho_lbm;
+    a.P = plane_stride(N, storage_bytes(p->storage));
+    switch (Q) {
+    case 15: fill_weights<tlbm::D3Q15>(a, p->exact_weights != 0); break;
+    case 19: fill_weights<tlbm::D3Q19>(a, p->exact_weights != 0); break;
+    default: fill_weights<tlbm::D3Q27>(a, p->exact_weights != 0); break;
+    }
+    auto cleanup_fail = [&](int code) {
+        tlbm_gpu_destroy(h);
+        return code;
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
+        return cleanup_fail(fail(TLBM_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
+    if ((e = hipEventCreate(&h->ev0)) != hipSuccess || (e = hipEventCreate(&h->ev1)) != hipSuccess)
+        return cleanup_fail(fail(TLBM_EHIP, std::string("hipEventCreate: ") + hipGetErrorString(e)));
+    const size_t fbytes = size_t(Q) * a.P * storage_bytes(p->storage);
+    for (int i = 0; i < 2; ++i) {
+        if ((e = hipMalloc(&h->f[i], fbytes)) != hipSuccess)
+            return cleanup_fail(fail(TLBM_ENOMEM, std::string("hipMalloc populations: ") +
+                                                      hipGetErrorString(e)));
+        h->bytes += fbytes;
+    }
+    if ((e = hipMalloc(&h->nd, N)) != hipSuccess)
+        return cleanup_fail(fail(TLBM_ENOMEM, std::string("hipMalloc ndType: ") + hipGetErrorString(e)));
+    h->bytes += N;
+    if ((e = hipMalloc(&h->uy_head, 32 * sizeof(double))) != hipSuccess)
+        return cleanup_fail(fail(TLBM_ENOMEM, "hipMalloc uy_head"));
+    h->nparts = int(std::min<uint32_t>(1024u, (N + tlbm::kBlock - 1) / tlbm::kBlock));
+    if ((e = hipMalloc(&h->partial, (size_t(h->nparts) * tlbm::kRed + 8) * sizeof(double))) != hipSuccess)
+        return cleanup_fail(fail(TLBM_ENOMEM, "hipMalloc reduction partials"));
+    if ((e = hipMemcpyAsync(h->nd, nd_type, N, hipMemcpyHostToDevice, h->stream)) != hipSuccess)
+        return cleanup_fail(fail(TLBM_EHIP, std::string("upload ndType: ") + hipGetErrorString(e)));
+    for (int i = 0; i < 2; ++i)
+        if ((e = h->ops->init_rest(Q, h->f[i], a, h->stream)) != hipSuccess)
+            return cleanup_fail(fail(TLBM_EHIP, std::string("init: ") + hipGetErrorString(e)));
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess)
+        return cleanup_fail(fail(TLBM_EHIP, std::string("init sync: ") + hipGetErrorString(e)));
+    h->cur = 0;
+    h->steps_since_set = 0;
+    *out = h;
+    return TLBM_OK;
+}
+
+int tlbm_gpu_destroy(tlbm_gpu *h) {
+    if (!h) return TLBM_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (int i = 0; i < 2; ++i)
+        if (h->f[i]) (void)hipFree(h->f[i]);
+    if (h->nd) (void)hipFree(h->nd);
+    if (h->macro) (void)hipFree(h->macro);
+    if (h->uy_head) (void)hipFree(h->uy_head);
+    if (h->scratch) (void)hipFree(h->scratch);
+    if (h->partial) (void)hipFree(h->partial);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return TLBM_OK;
+}
+
+static int queue_steps(tlbm_gpu *h, int nsteps) {
+    for (int k = 0; k < nsteps; ++k) {
+        const void *fin = h->f[h->cur];
+        void *fout = h->f[1 - h->cur];
+        if (h->has_lid) {
+            hipError_t e = h->ops->lbm(h->Q, tlbm::MODE_HEAD, fin, nullptr, h->nd, h->uy_head,
+                                       nullptr, h->args, h->stream);
+            if (e != hipSuccess) return fail(TLBM_EHIP, std::string("type-5 helper: ") + hipGetErrorString(e));
+        }
+        hipError_t e = h->ops->lbm(h->Q, tlbm::MODE_STEP, fin, fout, h->nd, nullptr, h->uy_head,
+                                   h->args, h->stream);
+        if (e != hipSuccess) return fail(TLBM_EHIP, std::string("stream-collide: ") + hipGetErrorString(e));
+        h->cur = 1 - h->cur;
+        ++h->steps_since_set;
+    }
+    return TLBM_OK;
+}
+
+int tlbm_gpu_step(tlbm_gpu *h, int nsteps) {
+    if (int rc = check(h)) return rc;
+    if (nsteps < 0) return fail(TLBM_EINVAL, "nsteps < 0");
+    return queue_steps(h, nsteps);
+}
+
+int tlbm_gpu_step_timed(tlbm_gpu *h, int nsteps, double *ms) {
+    if (int rc = check(h)) return rc;
+    if (nsteps < 0 || !ms) return fail(TLBM_EINVAL, "nsteps < 0 or null ms");
+    TLBM_HIP(hipEventRecord(h->ev0, h->stream));
+    if (int rc = queue_steps(h, nsteps)) return rc;
+    TLBM_HIP(hipEventRecord(h->ev1, h->stream));
+    TLBM_HIP(hipEventSynchronize(h->ev1));
+    float t = 0.f;
+    TLBM_HIP(hipEventElapsedTime(&t, h->ev0, h->ev1));
+    *ms = double(t);
+    return TLBM_OK;
+}
+
+int tlbm_gpu_sync(tlbm_gpu *h) {
+    if (int rc = check(h)) return rc;
+    TLBM_HIP(hipStreamSynchronize(h->stream));
+    return TLBM_OK;
+}
+
+// Copy `count` compute-type values from device `src` into host doubles.
+static int download_as_double(tlbm_gpu *h, const void *src, size_t count, double *const *dst,
+                              int nplanes) {
+    const size_t esz = size_t(h->ops->macro_bytes);
+    std::vector<unsigned char> tmp(esz * count * nplanes);
+    TLBM_HIP(hipMemcpyAsync(tmp.data(), src, tmp.size(), hipMemcpyDeviceToHost, h->stream));
+    TLBM_HIP(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < nplanes; ++k) {
+        if (!dst[k]) continue;
+        if (esz == 8) {
+            std::memcpy(dst[k], tmp.data() + size_t(k) * count * 8, count * 8);
+        } else {
+            const float *fp = reinterpret_cast<const float *>(tmp.data()) + size_t(k) * count;
+            for (size_t i = 0; i < count; ++i) dst[k][i] = double(fp[i]);
+        }
+    }
+    return TLBM_OK;
+}
+
+int tlbm_gpu_get_macros(tlbm_gpu *h, double *ux, double *uy, double *uz, double *rho) {
+    if (int rc = check(h)) return rc;
+    if (h->steps_since_set < 1)
+        return fail(TLBM_ESTATE, "moments exist only after a step (the reference computes them inside the step)");
+    if (!h->macro) {
+        const size_t b = size_t(4) * h->N * size_t(h->ops->macro_bytes);
+        TLBM_HIP(hipMalloc(&h->macro, b));
+        h->bytes += b;
+    }
+    // the last step read f[1-cur]; it is still intact (pull writes the other buffer)
+    hipError_t e = h->ops->lbm(h->Q, tlbm::MODE_MACROS, h->f[1 - h->cur], nullptr, h->nd,
+                               h->macro, nullptr, h->args, h->stream);
+    if (e != hipSuccess) return fail(TLBM_EHIP, std::string("moments: ") + hipGetErrorString(e));
+    double *dst[4] = {ux, uy, uz, rho};
+    return download_as_double(h, h->macro, h->N, dst, 4);
+}
+
+int tlbm_gpu_get_forces(tlbm_gpu *h, double *fx, double *fy, double *fz) {
+    if (int rc = check(h)) return rc;
+    if (!fx || !fy || !fz) return fail(TLBM_EINVAL, "null output");
+    if (int rc = ensure_scratch(h, size_t(3) * h->N * sizeof(double))) return rc;
+    hipError_t e = h->ops->forces(h->Q, h->f[h->cur], h->nd, h->scratch, h->args, h->stream);
+    if (e != hipSuccess) return fail(TLBM_EHIP, std::string("forces: ") + hipGetErrorString(e));
+    double *dst[3] = {fx, fy, fz};
+    for (int k = 0; k < 3; ++k)
+        TLBM_HIP(hipMemcpyAsync(dst[k], h->scratch + size_t(k) * h->N, size_t(h->N) * 8,
+                                hipMemcpyDeviceToHost, h->stream));
+    TLBM_HIP(hipStreamSynchronize(h->stream));
+    return TLBM_OK;
+}
+
+int tlbm_gpu_get_populations(tlbm_gpu *h, double *f) {
+    if (int rc = check(h)) return rc;
+    if (!f) return fail(TLBM_EINVAL, "null output");
+    const size_t b = size_t(h->N) * h->Q * sizeof(double);
+    if (int rc = ensure_scratch(h, b)) return rc;
+    hipError_t e = h->ops->get_populations(h->Q, h->f[h->cur], h->scratch, h->args, h->stream);
+    if (e != hipSuccess) return fail(TLBM_EHIP, std::string("get populations: ") + hipGetErrorString(e));
+    TLBM_HIP(hipMemcpyAsync(f, h->scratch, b, hipMemcpyDeviceToHost, h->stream));
+    TLBM_HIP(hipStreamSynchronize(h->stream));
+    return TLBM_OK;
+}
+
+int tlbm_gpu_set_populations(tlbm_gpu *h, const double *f) {
+    if (int rc = check(h)) return rc;
+    if (!f) return fail(TLBM_EINVAL, "null input");
+    const size_t b = size_t(h->N) * h->Q * sizeof(double);
+    if (int rc = ensure_scratch(h, b)) return rc;
+    TLBM_HIP(hipMemcpyAsync(h->scratch, f, b, hipMemcpyHostToDevice, h->stream));
+    for (int i = 0; i < 2; ++i) {
+        hipError_t e = h->ops->set_populations(h->Q, h->f[i], h->scratch, h->args, h->stream);
+        if (e != hipSuccess) return fail(TLBM_EHIP, std::string("set populations: ") + hipGetErrorString(e));
+    }
+    TLBM_HIP(hipStreamSynchronize(h->stream));
+    h->steps_since_set = 0;
+    return TLBM_OK;
+}
+
+int tlbm_gpu_init_from_macros(tlbm_gpu *h, const double *ux, const double *uy, const double *uz,
+                              const double *rho) {
+    if (int rc = check(h)) return rc;
+    if (!ux || !uy || !uz || !rho) return fail(TLBM_EINVAL, "null input");
+    const size_t plane = size_t(h->N) * sizeof(double);
+    if (int rc = ensure_scratch(h, 4 * plane)) return rc;
+    const double *src[4] = {ux, uy, uz, rho};
+    for (int k = 0; k < 4; ++k)
+        TLBM_HIP(hipMemcpyAsync(reinterpret_cast<char *>(h->scratch) + k * plane, src[k], plane,
+                                hipMemcpyHostToDevice, h->stream));
+    for (int i = 0; i < 2; ++i) {
+        hipError_t e = h->ops->init_macros(h->Q, h->f[i], h->scratch, h->args, h->stream);
+        if (e != hipSuccess) return fail(TLBM_EHIP, std::string("init from macros: ") + hipGetErrorString(e));
+    }
+    TLBM_HIP(hipStreamSynchronize(h->stream));
+    h->steps_since_set = 0;
+    return TLBM_OK;
+}
+
+int tlbm_gpu_reduce(tlbm_gpu *h, double out[5]) {
+    if (int rc = check(h)) return rc;
+    if (!out) return fail(TLBM_EINVAL, "null output");
+    const void *prev = h->steps_since_set >= 1 ? h->f[1 - h->cur] : nullptr;
+    double *res = h->partial + size_t(h->nparts) * tlbm::kRed;
+    hipError_t e = h->ops->reduce(h->Q, h->f[h->cur], prev, h->partial, h->nparts, res, h->args,
+                                  h->stream);
+    if (e != hipSuccess) return fail(TLBM_EHIP, std::string("reduce: ") + hipGetErrorString(e));
+    double r[tlbm::kRed];
+    TLBM_HIP(hipMemcpyAsync(r, res, sizeof r, hipMemcpyDeviceToHost, h->stream));
+    TLBM_HIP(hipStreamSynchronize(h->stream));
+    out[0] = r[0];
+    out[1] = r[1];
+    out[2] = r[2];
+    out[3] = r[3];
+    out[4] = (prev && r[5] > 0.0) ? std::sqrt(r[4] / r[5]) : 0.0;
+    return TLBM_OK;
+}
+
+int tlbm_gpu_memory_bytes(const tlbm_gpu *h, uint64_t *bytes) {
+    if (!h || !bytes) return fail(TLBM_EINVAL, "null argument");
+    *bytes = h->bytes;
+    return TLBM_OK;
+}
+
+}  // extern "C"
