@@ -15,24 +15,33 @@ struct Launch {
     using T = typename StorageT<ST>::T;
 
     template <class L>
-    static hipError_t lbm_l(int mode, const void *fin, void *fout, const uint8_t *nd, void *macro,
-                            const void *uy_head, const StepArgs &a, hipStream_t s) {
+    static hipError_t lbm_l(int mode, const void *fin, void *fout, const uint8_t *nd,
+                            const uint8_t *chunk_type, void *macro, const void *uy_head,
+                            const StepArgs &a, hipStream_t s) {
         const st *in = static_cast<const st *>(fin);
         st *out = static_cast<st *>(fout);
         T *mac = static_cast<T *>(macro);
         const T *uyh = static_cast<const T *>(uy_head);
+        using TU = Tune<ST>;
         switch (mode) {
         case MODE_STEP:
-            hipLaunchKernelGGL((k_lbm<L, ST, MODE_STEP>), dim3(blocks_for(a.N)), dim3(kBlock), 0, s,
-                               in, out, nd, mac, uyh, a);
+            if (TU::kVector && a.vec) {
+                constexpr uint32_t V = 16 / sizeof(st);
+                hipLaunchKernelGGL((k_lbm_vec<L, ST, 16>), dim3(blocks_for(a.N / V)), dim3(kBlock), 0,
+                                   s, in, out, nd, chunk_type, uyh, a);
+            } else {
+                hipLaunchKernelGGL((k_lbm<L, ST, MODE_STEP, false, TU::kNTLoad, true, TU::kBS>),
+                                   dim3((a.N + TU::kBS - 1) / TU::kBS), dim3(TU::kBS), 0, s, in, out,
+                                   nd, chunk_type, mac, uyh, a);
+            }
             break;
         case MODE_MACROS:
             hipLaunchKernelGGL((k_lbm<L, ST, MODE_MACROS>), dim3(blocks_for(a.N)), dim3(kBlock), 0,
-                               s, in, out, nd, mac, uyh, a);
+                               s, in, out, nd, nullptr, mac, uyh, a);
             break;
         case MODE_HEAD:
             hipLaunchKernelGGL((k_lbm<L, ST, MODE_HEAD>), dim3(1), dim3(64), 0, s, in, out, nd,
-                               mac, uyh, a);
+                               nullptr, mac, uyh, a);
             break;
         default:
             return hipErrorInvalidValue;
@@ -49,8 +58,9 @@ struct Launch {
     }
 
     static hipError_t lbm(int lattice, int mode, const void *fin, void *fout, const uint8_t *nd,
-                          void *macro, const void *uy_head, const StepArgs &a, hipStream_t s) {
-        TLBM_BY_LATTICE(lbm_l, mode, fin, fout, nd, macro, uy_head, a, s)
+                          const uint8_t *chunk_type, void *macro, const void *uy_head,
+                          const StepArgs &a, hipStream_t s) {
+        TLBM_BY_LATTICE(lbm_l, mode, fin, fout, nd, chunk_type, macro, uy_head, a, s)
     }
 
     template <class L>

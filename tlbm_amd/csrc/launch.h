@@ -13,7 +13,8 @@ namespace tlbm {
 
 struct LaunchOps {
     hipError_t (*lbm)(int lattice, int mode, const void *fin, void *fout, const uint8_t *nd,
-                      void *macro, const void *uy_head, const StepArgs &a, hipStream_t s);
+                      const uint8_t *chunk_type, void *macro, const void *uy_head,
+                      const StepArgs &a, hipStream_t s);
     hipError_t (*init_rest)(int lattice, void *fout, const StepArgs &a, hipStream_t s);
     hipError_t (*init_macros)(int lattice, void *fout, const double *mac, const StepArgs &a,
                               hipStream_t s);

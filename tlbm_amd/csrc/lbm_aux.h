@@ -66,7 +66,7 @@ k_init_macros(typename StorageT<ST>::st *__restrict__ fout, const double *__rest
         } else {
             v = feq_one<L, T>(s, S::weight(s, a), T(rho), T(ux), T(uy), T(uz));
         }
-        S::store(fout + size_t(s) * a.P + h, s, v, a);
+        S::store(fout + fidx<L::Q>(s, h, a), s, v, a);
     }
 }
 
@@ -86,7 +86,7 @@ k_init_rest(typename StorageT<ST>::st *__restrict__ fout, StepArgs a) {
             v = T(double(a.wf[s]) * a.rho_bc - double(a.wf[s]));
         else
             v = S::weight(s, a) * T(a.rho_bc);
-        S::store(fout + size_t(s) * a.P + h, s, v, a);
+        S::store(fout + fidx<L::Q>(s, h, a), s, v, a);
     }
 }
 
@@ -102,7 +102,7 @@ k_get_populations(const typename StorageT<ST>::st *__restrict__ fin, double *__r
 #pragma unroll
     for (int s = 0; s < L::Q; ++s) {
         const uint32_t src = c.shifted(-L::ex[s], -L::ey[s], -L::ez[s], a);
-        f_aos[size_t(g) * L::Q + s] = S::absolute(S::load(fin + size_t(s) * a.P + src, s, a), s, a);
+        f_aos[size_t(g) * L::Q + s] = S::absolute(S::load(fin + fidx<L::Q>(s, src, a), s, a), s, a);
     }
 }
 
@@ -118,7 +118,7 @@ k_set_populations(typename StorageT<ST>::st *__restrict__ fout, const double *__
 #pragma unroll
     for (int s = 0; s < L::Q; ++s) {
         const uint32_t src = c.shifted(L::ex[s], L::ey[s], L::ez[s], a);
-        S::store(fout + size_t(s) * a.P + h, s, S::from_absolute(f_aos[size_t(src) * L::Q + s], s, a), a);
+        S::store(fout + fidx<L::Q>(s, h, a), s, S::from_absolute(f_aos[size_t(src) * L::Q + s], s, a), a);
     }
 }
 
@@ -144,9 +144,9 @@ k_forces(const typename StorageT<ST>::st *__restrict__ fin, const uint8_t *__res
             if (nd_type[ngb] != 0) continue;
             const int b = L::bb[s];
             const double f1 = S::absolute(
-                S::load(fin + size_t(s) * a.P + c.shifted(-L::ex[s], -L::ey[s], -L::ez[s], a), s, a), s, a);
+                S::load(fin + fidx<L::Q>(s, c.shifted(-L::ex[s], -L::ey[s], -L::ez[s], a), a), s, a), s, a);
             const double f2 = S::absolute(
-                S::load(fin + size_t(b) * a.P + c.shifted(2 * L::ex[s], 2 * L::ey[s], 2 * L::ez[s], a), b, a), b, a);
+                S::load(fin + fidx<L::Q>(b, c.shifted(2 * L::ex[s], 2 * L::ey[s], 2 * L::ez[s], a), a), b, a), b, a);
             const double sum = f1 + f2;
             if (L::ex[b] > 0) fx += sum; else if (L::ex[b] < 0) fx -= sum;
             if (L::ey[b] > 0) fy += sum; else if (L::ey[b] < 0) fy -= sum;
@@ -171,14 +171,14 @@ k_reduce_partial(const typename StorageT<ST>::st *__restrict__ cur,
     for (uint32_t g = blockIdx.x * kBlock + threadIdx.x; g < a.N; g += gridDim.x * kBlock) {
 #pragma unroll
         for (int s = 0; s < L::Q; ++s) {
-            const double v = S::absolute(S::load(cur + size_t(s) * a.P + g, s, a), s, a);
+            const double v = S::absolute(S::load(cur + fidx<L::Q>(s, g, a), s, a), s, a);
             acc[0] += v;
             acc[1] += L::ex[s] * v;
             acc[2] += L::ey[s] * v;
             acc[3] += L::ez[s] * v;
             acc[5] += v * v;
             if (prev) {
-                const double d = v - S::absolute(S::load(prev + size_t(s) * a.P + g, s, a), s, a);
+                const double d = v - S::absolute(S::load(prev + fidx<L::Q>(s, g, a), s, a), s, a);
                 acc[4] += d * d;
             }
         }

@@ -1,10 +1,14 @@
 // lbm_kernels.h -- fused pull stream-collide kernel for gfx950 and the
 // per-node code it shares with the moment (dump) and type-5 helper passes.
 //
-// Layout in HBM: structure of arrays, one plane per direction s with a
-// stride of P >= N values: f[s*P + g], g = x + nx*y + nx*ny*z
-// (TLBM_Partition.cpp:730-737).  P is padded past N so that the 2*Q
-// streams of a step do not start at addresses a power of two apart.
+// Layout in HBM: chunked structure of arrays.  Nodes g = x + nx*y + nx*ny*z
+// (TLBM_Partition.cpp:730-737) are grouped in chunks of CH = 2^cb
+// consecutive nodes; a chunk stores its Q direction planes back to back:
+//   f(s, g) at ((g >> cb) * Q + s) * CH + (g & (CH - 1)).
+// Within a chunk the access is that of a plain structure of arrays (each
+// direction a contiguous, coalesced run); chunking keeps the 2*Q streams of
+// a step inside a few nearby DRAM regions, which the HBM serves closer to
+// copy speed than Q planes N values apart (measured on MI355X).
 // Two state buffers ping-pong.  A state buffer holds post-collision,
 // not-yet-streamed populations B_n; step n+1 gathers
 // A[g,s] = B_n[g - e_s (periodic), s], which is exactly what the reference's
@@ -43,7 +47,8 @@ constexpr int kRed = 6;      // values per FP64 reduction partial
 // Launch arguments, passed by value (kernarg segment, scalar registers).
 struct StepArgs {
     uint32_t nx, ny, nz, nxny, N;
-    uint32_t P;        // plane stride in values (>= N)
+    uint32_t cb;       // log2 of the chunk size CH (nodes)
+    int vec;           // 1: nx % V == 0, the step runs the 16-byte-per-lane kernel
     int dynamics;      // 1: BGK relaxation, other values: none (TLBM_Partition.cpp:795-807)
     int smag;          // cs > 0
     double cs, omega, nu, nu2, u_bc, rho_bc, inv_1mu, inv_rho_bc;
@@ -58,12 +63,20 @@ struct StepArgs {
     float drho_bc;     // rho_bc - 1
 };
 
+// Offset of population s of node g in a state buffer (chunked layout).
+template <int Q>
+__device__ inline size_t fidx(int s, uint32_t g, const StepArgs &a) {
+    return ((size_t(g >> a.cb) * Q + size_t(s)) << a.cb) + (g & ((1u << a.cb) - 1u));
+}
+
 // Raw value <-> compute value.  kShift: the compute value is d = f - w.
 template <int ST> struct StorageT;
 template <> struct StorageT<ST_F64> {
     using st = double;
     using T = double;
     static constexpr bool kShift = false;
+    __device__ static inline T in(st r, int, const StepArgs &) { return r; }
+    __device__ static inline st out(T v, int, const StepArgs &) { return v; }
     __device__ static inline T load(const st *p, int, const StepArgs &) { return *p; }
     __device__ static inline void store(st *p, int, T v, const StepArgs &) {
         __builtin_nontemporal_store(v, p);
@@ -76,6 +89,8 @@ template <> struct StorageT<ST_F32> {
     using st = float;
     using T = float;
     static constexpr bool kShift = true;
+    __device__ static inline T in(st r, int, const StepArgs &) { return r; }
+    __device__ static inline st out(T v, int, const StepArgs &) { return v; }
     __device__ static inline T load(const st *p, int, const StepArgs &) { return *p; }
     __device__ static inline void store(st *p, int, T v, const StepArgs &) {
         __builtin_nontemporal_store(v, p);
@@ -92,6 +107,8 @@ template <> struct StorageT<ST_F16> {
     using st = _Float16;
     using T = float;
     static constexpr bool kShift = true;
+    __device__ static inline T in(st r, int s, const StepArgs &a) { return float(r) * a.h_unscale[s]; }
+    __device__ static inline st out(T v, int s, const StepArgs &a) { return _Float16(v * a.h_scale[s]); }
     __device__ static inline T load(const st *p, int s, const StepArgs &a) {
         return float(*p) * a.h_unscale[s];
     }
@@ -105,6 +122,27 @@ template <> struct StorageT<ST_F16> {
     __device__ static inline T from_absolute(double f, int s, const StepArgs &a) {
         return T(f - double(a.wf[s]));
     }
+};
+
+// Launch choices per storage, measured on MI355X with the D3Q19
+// 128x128x512 lattice: FP64 runs the 16-byte vector kernel (two nodes per
+// lane); FP32 and FP16 run one node per lane in 512-thread workgroups, FP16
+// with non-temporal loads.  Every kernel stores non-temporally.
+template <int ST> struct Tune;
+template <> struct Tune<ST_F64> {
+    static constexpr bool kVector = true;
+    static constexpr int kBS = 1024;
+    static constexpr bool kNTLoad = true;
+};
+template <> struct Tune<ST_F32> {
+    static constexpr bool kVector = false;
+    static constexpr int kBS = 512;
+    static constexpr bool kNTLoad = false;
+};
+template <> struct Tune<ST_F16> {
+    static constexpr bool kVector = false;
+    static constexpr int kBS = 512;
+    static constexpr bool kNTLoad = true;
 };
 
 // Workgroup -> node block.  The dispatcher deals workgroups round-robin to
@@ -291,20 +329,27 @@ __device__ inline void node_update(T (&f)[L::Q], int t, T &rho, T &ux, T &uy, T 
 // fin -> fout; MODE_MACROS recomputes the moments of the step that read fin
 // (ux, uy, uz, rho planes of N values in `macro`); MODE_HEAD writes uy of
 // nodes 0..Q-1 (the values the type-5 rule reads) to `macro`.
-template <class L, int ST, int MODE>
-__global__ void __launch_bounds__(kBlock)
+// chunk_type (MODE_STEP, may be null): per chunk, the node type shared by
+// all its nodes, or 0xFF when they differ; a workgroup inside a uniform
+// chunk skips the per-node type load.
+template <class L, int ST, int MODE, bool REMAP = true, bool NTLOAD = false, bool NTSTORE = true,
+          int BS = kBlock>
+__global__ void __launch_bounds__(BS)
 k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::st *__restrict__ fout,
-      const uint8_t *__restrict__ nd_type, typename StorageT<ST>::T *__restrict__ macro,
-      const typename StorageT<ST>::T *__restrict__ uy_head, StepArgs a) {
+      const uint8_t *__restrict__ nd_type, const uint8_t *__restrict__ chunk_type,
+      typename StorageT<ST>::T *__restrict__ macro, const typename StorageT<ST>::T *__restrict__ uy_head,
+      StepArgs a) {
     using S = StorageT<ST>;
     using T = typename S::T;
+    using st = typename S::st;
     constexpr int Q = L::Q;
-    uint32_t g;
+    uint32_t g, blk = 0;
     if (MODE == MODE_HEAD) {
         g = threadIdx.x;
         if (g >= (uint32_t)Q || g >= a.N) return;
     } else {
-        g = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+        blk = REMAP ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+        g = blk * BS + threadIdx.x;
         if (g >= a.N) return;
     }
     const uint32_t z = g / a.nxny;
@@ -316,18 +361,30 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
     T f[Q];
 #pragma unroll
     for (int s = 0; s < Q; ++s) {
-        const size_t plane = size_t(s) * a.P;
-        f[s] = S::load(fin + plane + gat.src(L::ex[s], L::ey[s], L::ez[s]), s, a);
+        const st *p = fin + fidx<Q>(s, gat.src(L::ex[s], L::ey[s], L::ez[s]), a);
+        f[s] = S::in(NTLOAD ? __builtin_nontemporal_load(p) : *p, s, a);
     }
     T w[Q];
 #pragma unroll
     for (int s = 0; s < Q; ++s) w[s] = S::weight(s, a);
-    const int t = nd_type[g];
+    int t;
+    if (MODE == MODE_STEP && chunk_type) {
+        const uint8_t ct = chunk_type[(blk * BS) >> a.cb];  // BS divides the chunk
+        t = (ct != 0xFF) ? ct : nd_type[g];
+    } else {
+        t = nd_type[g];
+    }
     T rho, ux, uy, uz;
     if (MODE == MODE_STEP) {
         node_update<L, T, S::kShift, true>(f, t, rho, ux, uy, uz, a, uy_head, w);
 #pragma unroll
-        for (int s = 0; s < Q; ++s) S::store(fout + size_t(s) * a.P + g, s, f[s], a);
+        for (int s = 0; s < Q; ++s) {
+            st *p = fout + fidx<Q>(s, g, a);
+            if (NTSTORE)
+                __builtin_nontemporal_store(S::out(f[s], s, a), p);
+            else
+                *p = S::out(f[s], s, a);
+        }
     } else {
         node_update<L, T, S::kShift, false>(f, t, rho, ux, uy, uz, a, uy_head, w);
         if (MODE == MODE_MACROS) {
@@ -338,6 +395,144 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
         } else {
             macro[g] = uy;
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Vector path: each lane owns V = VB/sizeof(st) consecutive x-nodes, so every
+// population plane moves as one VB-byte load and one VB-byte store per lane
+// (non-temporal).  The pull gather shifts a plane by -e_x along x; the one
+// element a lane is missing sits in the neighbouring lane's vector and
+// arrives by a DPP wavefront shift (wave_shr:1 / wave_shl:1: lane i reads
+// lane i-1 / i+1).  Lanes at a wavefront edge or at the periodic x wrap load
+// that element instead.  Requires nx % V == 0 (so rows, chunks and vectors
+// align); whole waves stay active through the shifts.
+
+template <int VB> struct RawVec;
+template <> struct RawVec<8> { typedef uint32_t type __attribute__((ext_vector_type(2))); };
+template <> struct RawVec<16> { typedef uint32_t type __attribute__((ext_vector_type(4))); };
+
+__device__ inline uint32_t dpp_from_prev_lane(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ inline uint32_t dpp_from_next_lane(uint32_t v) {
+    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x130, 0xF, 0xF, false));
+}
+
+// Value of `v` in the lane before (PREV) or after this one.
+template <bool PREV, class st>
+__device__ inline st lane_shift(st v) {
+    if constexpr (sizeof(st) == 8) {
+        uint64_t b;
+        __builtin_memcpy(&b, &v, 8);
+        const uint32_t lo = PREV ? dpp_from_prev_lane(uint32_t(b)) : dpp_from_next_lane(uint32_t(b));
+        const uint32_t hi = PREV ? dpp_from_prev_lane(uint32_t(b >> 32)) : dpp_from_next_lane(uint32_t(b >> 32));
+        b = (uint64_t(hi) << 32) | lo;
+        st r;
+        __builtin_memcpy(&r, &b, 8);
+        return r;
+    } else if constexpr (sizeof(st) == 4) {
+        uint32_t b;
+        __builtin_memcpy(&b, &v, 4);
+        b = PREV ? dpp_from_prev_lane(b) : dpp_from_next_lane(b);
+        st r;
+        __builtin_memcpy(&r, &b, 4);
+        return r;
+    } else {
+        uint16_t h;
+        __builtin_memcpy(&h, &v, 2);
+        const uint32_t b = PREV ? dpp_from_prev_lane(uint32_t(h)) : dpp_from_next_lane(uint32_t(h));
+        h = uint16_t(b);
+        st r;
+        __builtin_memcpy(&r, &h, 2);
+        return r;
+    }
+}
+
+template <int V> struct NodeTypesT;
+template <> struct NodeTypesT<2> { using type = uint16_t; };
+template <> struct NodeTypesT<4> { using type = uint32_t; };
+template <> struct NodeTypesT<8> { using type = uint64_t; };
+
+template <class L, int ST, int VB, int MINW = 1>
+__global__ void __launch_bounds__(kBlock, MINW)
+k_lbm_vec(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::st *__restrict__ fout,
+          const uint8_t *__restrict__ nd_type, const uint8_t *__restrict__ chunk_type,
+          const typename StorageT<ST>::T *__restrict__ uy_head, StepArgs a) {
+    using S = StorageT<ST>;
+    using T = typename S::T;
+    using st = typename S::st;
+    using raw_t = typename RawVec<VB>::type;
+    constexpr int Q = L::Q;
+    constexpr int V = VB / int(sizeof(st));
+    static_assert(V >= 2, "a vector lane holds at least two nodes");
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    const bool active = t < a.N / V;
+    const uint32_t g0 = active ? t * V : 0;  // idle lanes still feed the lane shifts
+    const uint32_t z = g0 / a.nxny;
+    const uint32_t r = g0 - z * a.nxny;
+    const uint32_t y = r / a.nx;
+    const uint32_t x0 = r - y * a.nx;
+    Gather gat;
+    gat.init(x0, y, z, a);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t xe = x0 + V - 1;
+    const uint32_t x_prev = gat.xs[2];                     // x0 - 1, periodic
+    const uint32_t x_next = (xe + 1 == a.nx) ? 0 : xe + 1;  // xe + 1, periodic
+    const bool own_prev = lane == 0 || x0 == 0;
+    const bool own_next = lane == 63 || xe + 1 == a.nx;
+    st e[Q][V];
+#pragma unroll
+    for (int s = 0; s < Q; ++s) {
+        const int ex = L::ex[s];
+        const uint32_t row = gat.ys[L::ey[s] + 1] + gat.zs[L::ez[s] + 1];
+        const raw_t raw = __builtin_nontemporal_load(
+            reinterpret_cast<const raw_t *>(fin + fidx<Q>(s, row + x0, a)));
+        st v[V];
+        __builtin_memcpy(v, &raw, VB);
+        if (ex == 0) {
+#pragma unroll
+            for (int k = 0; k < V; ++k) e[s][k] = v[k];
+        } else if (ex > 0) {  // elements x0-1 .. xe-1
+            st p = lane_shift<true>(v[V - 1]);
+            if (own_prev) p = fin[fidx<Q>(s, row + x_prev, a)];
+            e[s][0] = p;
+#pragma unroll
+            for (int k = 1; k < V; ++k) e[s][k] = v[k - 1];
+        } else {  // elements x0+1 .. xe+1
+            st n = lane_shift<false>(v[0]);
+            if (own_next) n = fin[fidx<Q>(s, row + x_next, a)];
+#pragma unroll
+            for (int k = 0; k < V - 1; ++k) e[s][k] = v[k + 1];
+            e[s][V - 1] = n;
+        }
+    }
+    if (!active) return;
+    typename NodeTypesT<V>::type types;
+    const uint8_t ct = chunk_type ? chunk_type[(blockIdx.x * kBlock * V) >> a.cb] : uint8_t(0xFF);
+    if (ct != 0xFF)
+        types = typename NodeTypesT<V>::type(ct) * (~typename NodeTypesT<V>::type(0) / 0xFFu);
+    else
+        __builtin_memcpy(&types, nd_type + g0, sizeof(types));
+    T w[Q];
+#pragma unroll
+    for (int s = 0; s < Q; ++s) w[s] = S::weight(s, a);
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+        T f[Q];
+#pragma unroll
+        for (int s = 0; s < Q; ++s) f[s] = S::in(e[s][k], s, a);
+        T rho, ux, uy, uz;
+        const int tk = int((types >> (8 * k)) & 0xFFu);
+        node_update<L, T, S::kShift, true>(f, tk, rho, ux, uy, uz, a, uy_head, w);
+#pragma unroll
+        for (int s = 0; s < Q; ++s) e[s][k] = S::out(f[s], s, a);
+    }
+#pragma unroll
+    for (int s = 0; s < Q; ++s) {
+        raw_t o;
+        __builtin_memcpy(&o, e[s], VB);
+        __builtin_nontemporal_store(o, reinterpret_cast<raw_t *>(fout + fidx<Q>(s, g0, a)));
     }
 }
 

@@ -51,11 +51,12 @@ void fill_weights(tlbm::StepArgs &a, bool exact) {
     a.drho_bc = float(a.rho_bc - 1.0);
 }
 
-// Plane stride: N rounded up to whole 256-byte lines plus an odd number of
-// lines, so the 2*Q streams of a step never start a power of two apart.
-uint32_t plane_stride(uint32_t N, size_t value_bytes) {
-    const uint32_t line = uint32_t(256 / value_bytes);
-    return ((N + line - 1) / line) * line + 3 * line;
+// Chunk of the layout: 32 KiB of one direction plane (8192 FP32 values),
+// the size that ran the D3Q19 gather/scatter fastest on MI355X.
+uint32_t chunk_bits(size_t value_bytes) {
+    uint32_t cb = 0;
+    while ((size_t(1) << (cb + 1)) * value_bytes <= 32768) ++cb;
+    return cb;
 }
 
 int lattice_q(int lattice) {
@@ -85,6 +86,7 @@ struct tlbm_gpu {
     void *f[2] = {nullptr, nullptr};
     int cur = 0;              // f[cur] holds the state the next step reads from
     uint8_t *nd = nullptr;
+    uint8_t *chunk_type = nullptr;  // per chunk: common node type or 0xFF
     void *macro = nullptr;    // 4*N compute-type values (dumps)
     void *uy_head = nullptr;  // Q compute-type values (type-5 rule)
     double *scratch = nullptr;  // staging for forces / populations / macros (lazy)
@@ -96,6 +98,7 @@ struct tlbm_gpu {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t bytes = 0;
+    std::vector<uint8_t> chunk_types;  // host copy, kept until the upload completes
 };
 
 namespace {
@@ -185,7 +188,8 @@ int tlbm_gpu_create(const tlbm_params *p, const uint8_t *nd_type, tlbm_gpu **out
     a.rho_bc = p->rho_lbm;
     a.inv_1mu = 1. / (1. - p->u_lbm);
     a.inv_rho_bc = 1. / p->rho_lbm;
-    a.P = plane_stride(N, storage_bytes(p->storage));
+    a.cb = chunk_bits(storage_bytes(p->storage));
+    a.vec = (a.nx % uint32_t(16 / storage_bytes(p->storage))) == 0;
     switch (Q) {
     case 15: fill_weights<tlbm::D3Q15>(a, p->exact_weights != 0); break;
     case 19: fill_weights<tlbm::D3Q19>(a, p->exact_weights != 0); break;
@@ -200,7 +204,8 @@ int tlbm_gpu_create(const tlbm_params *p, const uint8_t *nd_type, tlbm_gpu **out
         return cleanup_fail(fail(TLBM_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
     if ((e = hipEventCreate(&h->ev0)) != hipSuccess || (e = hipEventCreate(&h->ev1)) != hipSuccess)
         return cleanup_fail(fail(TLBM_EHIP, std::string("hipEventCreate: ") + hipGetErrorString(e)));
-    const size_t fbytes = size_t(Q) * a.P * storage_bytes(p->storage);
+    const size_t nchunks = (size_t(N) + (size_t(1) << a.cb) - 1) >> a.cb;
+    const size_t fbytes = (nchunks * Q << a.cb) * storage_bytes(p->storage);
     for (int i = 0; i < 2; ++i) {
         if ((e = hipMalloc(&h->f[i], fbytes)) != hipSuccess)
             return cleanup_fail(fail(TLBM_ENOMEM, std::string("hipMalloc populations: ") +
@@ -217,6 +222,24 @@ int tlbm_gpu_create(const tlbm_params *p, const uint8_t *nd_type, tlbm_gpu **out
         return cleanup_fail(fail(TLBM_ENOMEM, "hipMalloc reduction partials"));
     if ((e = hipMemcpyAsync(h->nd, nd_type, N, hipMemcpyHostToDevice, h->stream)) != hipSuccess)
         return cleanup_fail(fail(TLBM_EHIP, std::string("upload ndType: ") + hipGetErrorString(e)));
+    {
+        // per-chunk common node type (0xFF: mixed), read once per workgroup
+        const uint32_t CH = 1u << a.cb;
+        h->chunk_types.assign(nchunks, 0xFF);
+        for (size_t c = 0; c < nchunks; ++c) {
+            const uint32_t g0 = uint32_t(c * CH), g1 = std::min<uint32_t>(N, g0 + CH);
+            uint8_t t0 = nd_type[g0];
+            for (uint32_t g = g0 + 1; g < g1 && t0 != 0xFF; ++g)
+                if (nd_type[g] != t0) t0 = 0xFF;
+            h->chunk_types[c] = t0;
+        }
+        if ((e = hipMalloc(&h->chunk_type, nchunks)) != hipSuccess)
+            return cleanup_fail(fail(TLBM_ENOMEM, "hipMalloc chunk types"));
+        h->bytes += nchunks;
+        if ((e = hipMemcpyAsync(h->chunk_type, h->chunk_types.data(), nchunks, hipMemcpyHostToDevice,
+                                h->stream)) != hipSuccess)
+            return cleanup_fail(fail(TLBM_EHIP, std::string("upload chunk types: ") + hipGetErrorString(e)));
+    }
     for (int i = 0; i < 2; ++i)
         if ((e = h->ops->init_rest(Q, h->f[i], a, h->stream)) != hipSuccess)
             return cleanup_fail(fail(TLBM_EHIP, std::string("init: ") + hipGetErrorString(e)));
@@ -234,6 +257,7 @@ int tlbm_gpu_destroy(tlbm_gpu *h) {
     for (int i = 0; i < 2; ++i)
         if (h->f[i]) (void)hipFree(h->f[i]);
     if (h->nd) (void)hipFree(h->nd);
+    if (h->chunk_type) (void)hipFree(h->chunk_type);
     if (h->macro) (void)hipFree(h->macro);
     if (h->uy_head) (void)hipFree(h->uy_head);
     if (h->scratch) (void)hipFree(h->scratch);
@@ -250,12 +274,12 @@ static int queue_steps(tlbm_gpu *h, int nsteps) {
         const void *fin = h->f[h->cur];
         void *fout = h->f[1 - h->cur];
         if (h->has_lid) {
-            hipError_t e = h->ops->lbm(h->Q, tlbm::MODE_HEAD, fin, nullptr, h->nd, h->uy_head,
-                                       nullptr, h->args, h->stream);
+            hipError_t e = h->ops->lbm(h->Q, tlbm::MODE_HEAD, fin, nullptr, h->nd, nullptr,
+                                       h->uy_head, nullptr, h->args, h->stream);
             if (e != hipSuccess) return fail(TLBM_EHIP, std::string("type-5 helper: ") + hipGetErrorString(e));
         }
-        hipError_t e = h->ops->lbm(h->Q, tlbm::MODE_STEP, fin, fout, h->nd, nullptr, h->uy_head,
-                                   h->args, h->stream);
+        hipError_t e = h->ops->lbm(h->Q, tlbm::MODE_STEP, fin, fout, h->nd, h->chunk_type, nullptr,
+                                   h->uy_head, h->args, h->stream);
         if (e != hipSuccess) return fail(TLBM_EHIP, std::string("stream-collide: ") + hipGetErrorString(e));
         h->cur = 1 - h->cur;
         ++h->steps_since_set;
@@ -317,7 +341,7 @@ int tlbm_gpu_get_macros(tlbm_gpu *h, double *ux, double *uy, double *uz, double 
         h->bytes += b;
     }
     // the last step read f[1-cur]; it is still intact (pull writes the other buffer)
-    hipError_t e = h->ops->lbm(h->Q, tlbm::MODE_MACROS, h->f[1 - h->cur], nullptr, h->nd,
+    hipError_t e = h->ops->lbm(h->Q, tlbm::MODE_MACROS, h->f[1 - h->cur], nullptr, h->nd, nullptr,
                                h->macro, nullptr, h->args, h->stream);
     if (e != hipSuccess) return fail(TLBM_EHIP, std::string("moments: ") + hipGetErrorString(e));
     double *dst[4] = {ux, uy, uz, rho};
