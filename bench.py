@@ -146,6 +146,30 @@ def cpu_baseline(nd, args):
                           f"lattice for {steps} steps (reference run unavailable: {exc})"}
 
 
+def copy_calibration(mib, reps=5):
+    """Device-to-device copies of `mib` MiB through the HIP runtime (a plain
+    copy kernel of known size): the reference point for scaling the
+    FETCH_SIZE / WRITE_SIZE counters of the stream-collide kernel."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = int(mib) << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for buf in (a, b):
+        if hip.hipMalloc(ctypes.byref(buf), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(a, 1, ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMemset failed")
+        for _ in range(reps):
+            if hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+                raise RuntimeError("hipMemcpy failed")
+        hip.hipDeviceSynchronize()
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    print(json.dumps({"copy_bytes": n, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -157,7 +181,12 @@ def main():
     ap.add_argument("--cpu-timeout", type=float, default=240.0)
     ap.add_argument("--storages", default="f32,f64,f16",
                     help="storages to run; the first is the headline")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the lattice lives whole on one MI355X and this build has "
               f"no multi-GPU path (asked for --gpus {args.gpus})", file=sys.stderr)
