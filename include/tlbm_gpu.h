@@ -109,6 +109,19 @@ int tlbm_gpu_init_from_macros(tlbm_gpu *h, const double *ux, const double *uy, c
  * no conservation or residual checks. */
 int tlbm_gpu_reduce(tlbm_gpu *h, double out[5]);
 
+/* enable = 1: zero the time-average sums and from then on add ux, uy, uz
+ * and rho of every step to them on the device; enable = 0: stop adding.
+ * TLBM_Partition::update_time_avg (src/TLBM_Partition.cpp:815-824) driven
+ * by Problem::timeAvgFlag (the reference reads an uninitialised flag).
+ * The sums are FP64. */
+int tlbm_gpu_time_average(tlbm_gpu *h, int enable);
+
+/* The time-average sums (not divided by the step count, as
+ * TLBM_Partition::write_time_avg_data writes them, src/TLBM_Partition.cpp:
+ * 564-606) and the number of steps summed.  Arrays of nx*ny*nz doubles. */
+int tlbm_gpu_get_time_average(tlbm_gpu *h, double *u, double *v, double *w, double *rho,
+                              int64_t *nsteps);
+
 /* Bytes of device memory the handle holds. */
 int tlbm_gpu_memory_bytes(const tlbm_gpu *h, uint64_t *bytes);
 

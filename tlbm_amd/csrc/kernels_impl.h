@@ -20,7 +20,7 @@ struct Launch {
                             const StepArgs &a, hipStream_t s) {
         const st *in = static_cast<const st *>(fin);
         st *out = static_cast<st *>(fout);
-        T *mac = static_cast<T *>(macro);
+        void *mac = macro;
         const T *uyh = static_cast<const T *>(uy_head);
         using TU = Tune<ST>;
         switch (mode) {
@@ -37,6 +37,10 @@ struct Launch {
             break;
         case MODE_MACROS:
             hipLaunchKernelGGL((k_lbm<L, ST, MODE_MACROS>), dim3(blocks_for(a.N)), dim3(kBlock), 0,
+                               s, in, out, nd, nullptr, mac, uyh, a);
+            break;
+        case MODE_AVG:
+            hipLaunchKernelGGL((k_lbm<L, ST, MODE_AVG>), dim3(blocks_for(a.N)), dim3(kBlock), 0,
                                s, in, out, nd, nullptr, mac, uyh, a);
             break;
         case MODE_HEAD:

@@ -39,7 +39,7 @@
 namespace tlbm {
 
 enum Storage : int { ST_F64 = 0, ST_F32 = 1, ST_F16 = 2 };
-enum Mode : int { MODE_STEP = 0, MODE_MACROS = 1, MODE_HEAD = 2 };
+enum Mode : int { MODE_STEP = 0, MODE_MACROS = 1, MODE_HEAD = 2, MODE_AVG = 3 };
 
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
 constexpr int kRed = 6;      // values per FP64 reduction partial
@@ -327,8 +327,10 @@ __device__ inline void node_update(T (&f)[L::Q], int t, T &rho, T &ux, T &uy, T 
 
 // One lattice update per thread.  MODE_STEP streams and collides
 // fin -> fout; MODE_MACROS recomputes the moments of the step that read fin
-// (ux, uy, uz, rho planes of N values in `macro`); MODE_HEAD writes uy of
-// nodes 0..Q-1 (the values the type-5 rule reads) to `macro`.
+// (ux, uy, uz, rho planes of N compute-type values in `out`); MODE_AVG adds
+// those moments to four planes of N doubles in `out` (time average);
+// MODE_HEAD writes uy of nodes 0..Q-1 (the values the type-5 rule reads) to
+// `out`.
 // chunk_type (MODE_STEP, may be null): per chunk, the node type shared by
 // all its nodes, or 0xFF when they differ; a workgroup inside a uniform
 // chunk skips the per-node type load.
@@ -337,8 +339,7 @@ template <class L, int ST, int MODE, bool REMAP = true, bool NTLOAD = false, boo
 __global__ void __launch_bounds__(BS)
 k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::st *__restrict__ fout,
       const uint8_t *__restrict__ nd_type, const uint8_t *__restrict__ chunk_type,
-      typename StorageT<ST>::T *__restrict__ macro, const typename StorageT<ST>::T *__restrict__ uy_head,
-      StepArgs a) {
+      void *__restrict__ out, const typename StorageT<ST>::T *__restrict__ uy_head, StepArgs a) {
     using S = StorageT<ST>;
     using T = typename S::T;
     using st = typename S::st;
@@ -388,12 +389,19 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
     } else {
         node_update<L, T, S::kShift, false>(f, t, rho, ux, uy, uz, a, uy_head, w);
         if (MODE == MODE_MACROS) {
+            T *macro = static_cast<T *>(out);
             macro[g] = ux;
             macro[size_t(a.N) + g] = uy;
             macro[2 * size_t(a.N) + g] = uz;
             macro[3 * size_t(a.N) + g] = rho;
+        } else if (MODE == MODE_AVG) {
+            double *sum = static_cast<double *>(out);
+            sum[g] += double(ux);
+            sum[size_t(a.N) + g] += double(uy);
+            sum[2 * size_t(a.N) + g] += double(uz);
+            sum[3 * size_t(a.N) + g] += double(rho);
         } else {
-            macro[g] = uy;
+            static_cast<T *>(out)[g] = uy;
         }
     }
 }

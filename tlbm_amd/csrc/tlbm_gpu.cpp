@@ -99,6 +99,9 @@ struct tlbm_gpu {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t bytes = 0;
     std::vector<uint8_t> chunk_types;  // host copy, kept until the upload completes
+    double *avg = nullptr;             // 4*N FP64 time-average sums
+    bool avg_on = false;
+    int64_t avg_steps = 0;
 };
 
 namespace {
@@ -258,6 +261,7 @@ int tlbm_gpu_destroy(tlbm_gpu *h) {
         if (h->f[i]) (void)hipFree(h->f[i]);
     if (h->nd) (void)hipFree(h->nd);
     if (h->chunk_type) (void)hipFree(h->chunk_type);
+    if (h->avg) (void)hipFree(h->avg);
     if (h->macro) (void)hipFree(h->macro);
     if (h->uy_head) (void)hipFree(h->uy_head);
     if (h->scratch) (void)hipFree(h->scratch);
@@ -283,6 +287,13 @@ static int queue_steps(tlbm_gpu *h, int nsteps) {
         if (e != hipSuccess) return fail(TLBM_EHIP, std::string("stream-collide: ") + hipGetErrorString(e));
         h->cur = 1 - h->cur;
         ++h->steps_since_set;
+        if (h->avg_on) {
+            // the step just read f[1-cur]; its moments go into the sums
+            e = h->ops->lbm(h->Q, tlbm::MODE_AVG, h->f[1 - h->cur], nullptr, h->nd, nullptr, h->avg,
+                            nullptr, h->args, h->stream);
+            if (e != hipSuccess) return fail(TLBM_EHIP, std::string("time average: ") + hipGetErrorString(e));
+            ++h->avg_steps;
+        }
     }
     return TLBM_OK;
 }
@@ -424,6 +435,35 @@ int tlbm_gpu_reduce(tlbm_gpu *h, double out[5]) {
     out[2] = r[2];
     out[3] = r[3];
     out[4] = (prev && r[5] > 0.0) ? std::sqrt(r[4] / r[5]) : 0.0;
+    return TLBM_OK;
+}
+
+int tlbm_gpu_time_average(tlbm_gpu *h, int enable) {
+    if (int rc = check(h)) return rc;
+    if (enable) {
+        const size_t b = size_t(4) * h->N * sizeof(double);
+        if (!h->avg) {
+            TLBM_HIP(hipMalloc(&h->avg, b));
+            h->bytes += b;
+        }
+        TLBM_HIP(hipMemsetAsync(h->avg, 0, b, h->stream));
+        h->avg_steps = 0;
+    }
+    h->avg_on = enable != 0;
+    return TLBM_OK;
+}
+
+int tlbm_gpu_get_time_average(tlbm_gpu *h, double *u, double *v, double *w, double *rho,
+                              int64_t *nsteps) {
+    if (int rc = check(h)) return rc;
+    if (!h->avg) return fail(TLBM_ESTATE, "time averaging was never enabled");
+    double *dst[4] = {u, v, w, rho};
+    for (int k = 0; k < 4; ++k)
+        if (dst[k])
+            TLBM_HIP(hipMemcpyAsync(dst[k], h->avg + size_t(k) * h->N, size_t(h->N) * sizeof(double),
+                                    hipMemcpyDeviceToHost, h->stream));
+    TLBM_HIP(hipStreamSynchronize(h->stream));
+    if (nsteps) *nsteps = h->avg_steps;
     return TLBM_OK;
 }
 

@@ -60,11 +60,13 @@ def load_library(path: str = LIB_PATH):
     L.tlbm_gpu_init_from_macros.argtypes = [_P, _P, _P, _P, _P]
     L.tlbm_gpu_reduce.argtypes = [_P, _P]
     L.tlbm_gpu_memory_bytes.argtypes = [_P, ctypes.POINTER(ctypes.c_uint64)]
+    L.tlbm_gpu_time_average.argtypes = [_P, ctypes.c_int]
+    L.tlbm_gpu_get_time_average.argtypes = [_P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_int64)]
     L.tlbm_gpu_last_error.restype = ctypes.c_char_p
     L.tlbm_gpu_last_error.argtypes = []
     for name in ("create", "destroy", "step", "step_timed", "sync", "get_macros", "get_forces",
                  "get_populations", "set_populations", "init_from_macros", "reduce",
-                 "memory_bytes"):
+                 "memory_bytes", "time_average", "get_time_average"):
         getattr(L, f"tlbm_gpu_{name}").restype = ctypes.c_int
     _lib = L
     return L
@@ -162,6 +164,16 @@ class Lattice:
         out = np.zeros(5, np.float64)
         _check(load_library().tlbm_gpu_reduce(self._h, out.ctypes.data))
         return {"mass": out[0], "momentum": out[1:4].copy(), "residual": out[4]}
+
+    def time_average(self, enable: bool = True) -> None:
+        _check(load_library().tlbm_gpu_time_average(self._h, int(bool(enable))))
+
+    def get_time_average(self):
+        out = [np.empty(self.N, np.float64) for _ in range(4)]
+        n = ctypes.c_int64()
+        _check(load_library().tlbm_gpu_get_time_average(self._h, *[a.ctypes.data for a in out],
+                                                          ctypes.byref(n)))
+        return dict(zip(("u", "v", "w", "rho"), out)), int(n.value)
 
     def memory_bytes(self) -> int:
         b = ctypes.c_uint64()
