@@ -122,6 +122,55 @@ int tlbm_gpu_time_average(tlbm_gpu *h, int enable);
 int tlbm_gpu_get_time_average(tlbm_gpu *h, double *u, double *v, double *w, double *rho,
                               int64_t *nsteps);
 
+/* Inlet velocity per inlet column: type-2 nodes at (x, y) take
+ * u_z = uz[x + nx*y] (nx*ny doubles, stored as float) instead of u_lbm
+ * (uz = NULL: back to u_lbm).  No reference counterpart: the reference's
+ * inlet is uniform (D3Q19LatticeStructure.hpp:108-132). */
+int tlbm_gpu_set_inlet_profile(tlbm_gpu *h, const double *uz);
+
+/* ---- learned closure (no reference counterpart) --------------------------
+ * A network of TLBM_MLP_NPARAM floats -- W1[32][7], b1[32], W2[32][32], b2[32],
+ * w3[32], b3 -- maps a node's non-equilibrium stress (xx, yy, zz, xy, xz, yz
+ * and its norm, the inputs of the Smagorinsky rule of
+ * include/LatticeStructure.hpp:140-153), each times feature_scale, through
+ * relu(32) -> relu(32) -> z to the relaxation time
+ * tau = 1/omega + output_scale * softplus(z).  It runs inside the collide
+ * kernel on bf16 MFMA (FP32 accumulation). */
+#define TLBM_MLP_NPARAM 1345
+
+/* Use the network in place of Smagorinsky for the following steps
+ * (params = NULL: back to the Smagorinsky / BGK rule).  FP32/FP16 storage. */
+int tlbm_gpu_set_closure(tlbm_gpu *h, const float *params, double feature_scale,
+                         double output_scale);
+
+/* 2x2x2 box filter of the last step's moments onto the (nx/2, ny/2, nz/2)
+ * lattice (all even): 10 planes of doubles -- rho, ux, uy, uz, uxux, uyuy,
+ * uzuz, uxuy, uxuz, uyuz (means of the 8 fine nodes), coarse global order. */
+int tlbm_gpu_box_filter(tlbm_gpu *h, double *out);
+
+typedef struct tlbm_mlp_trainer tlbm_mlp_trainer;
+
+/* A training set of n samples (x: n*7 floats, t: n targets of
+ * softplus(z)) on the device, from initial parameters; batches of at most
+ * max_batch samples. */
+int tlbm_mlp_trainer_create(const float *x, const float *t, int64_t n, const float *init,
+                            uint64_t seed, int max_batch, tlbm_mlp_trainer **out);
+/* steps Adam steps (beta 0.9/0.999, eps 1e-8) on the mean of
+ * (softplus(z) - t)^2 / 2 over batches of `batch` hashed sample indices;
+ * last_loss (may be NULL) gets the last batch's loss. */
+int tlbm_mlp_trainer_step(tlbm_mlp_trainer *tr, int steps, int batch, float lr,
+                          double *last_loss);
+/* Loss and gradient of samples [first, first+count) at the current
+ * parameters (no update). */
+int tlbm_mlp_trainer_loss_grad(tlbm_mlp_trainer *tr, int64_t first, int64_t count, double *loss,
+                               float *grad);
+int tlbm_mlp_trainer_params(tlbm_mlp_trainer *tr, float *params);
+int tlbm_mlp_trainer_destroy(tlbm_mlp_trainer *tr);
+
+/* z of n feature vectors through the collide kernel's wavefront inference
+ * (the same device code), for checking it. */
+int tlbm_mlp_infer(const float *params, const float *x, int64_t n, float *z);
+
 /* Bytes of device memory the handle holds. */
 int tlbm_gpu_memory_bytes(const tlbm_gpu *h, uint64_t *bytes);
 

@@ -18,6 +18,8 @@
 #endif
 #include "oracle_lattice.h"
 
+void mlp_infer_bf16(const float *params, const float *x, int n, double *z_out);
+
 #define REAL double
 #define FN(name) name##_f64
 #include "lbm_oracle_impl.h"

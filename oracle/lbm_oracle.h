@@ -68,6 +68,13 @@ void oracle_get_populations_f32(const oracle_f32 *o, float *f);
 void oracle_set_populations_f64(oracle_f64 *o, const double *f);
 void oracle_set_populations_f32(oracle_f32 *o, const float *f);
 
+/* Learned closure in place of Smagorinsky: the network (1345 floats, the
+ * layout of tlbm_gpu.h) on the stress features times cl_in, inferred as the
+ * collide kernel does (bf16), tau = 1/omega + cl_out*softplus(z).
+ * params = NULL switches it off. */
+void oracle_set_closure_f64(oracle_f64 *o, const float *params, double cl_in, double cl_out);
+void oracle_set_closure_f32(oracle_f32 *o, const float *params, double cl_in, double cl_out);
+
 /* Weight table the oracle uses (Q entries), for table checks. */
 void oracle_get_weights_f64(const oracle_f64 *o, double *w);
 

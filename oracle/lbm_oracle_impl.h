@@ -22,6 +22,9 @@ struct FN(oracle) {
     REAL *ux, *uy, *uz, *rho;
     REAL *fOut;        /* buffer written by the last step */
     long ts;
+    int closure;       /* learned closure in place of Smagorinsky */
+    float mlp[1345];
+    double cl_in, cl_out;
 };
 
 /* LatticeStructure.hpp:92-103 */
@@ -150,7 +153,26 @@ static void FN(process_node)(struct FN(oracle) *o, REAL *fOut, REAL *fIn, long n
         }
 
         REAL omega = o->omega;
-        if (o->cs > 0) {
+        if (o->closure) {
+            /* learned closure: the stress of compute_strain_tensor and its
+             * norm, scaled, through the network (bf16 inference) */
+            double S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int s = 0; s < Q; ++s) {
+                const double e3[3] = {L->ex[s], L->ey[s], L->ez[s]};
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) S[i * 3 + j] += e3[i] * e3[j] * ((double)f[s] - (double)feq[s]);
+            }
+            const double qn = sqrt(S[0] * S[0] + S[4] * S[4] + S[8] * S[8] +
+                                   2.0 * (S[1] * S[1] + S[2] * S[2] + S[5] * S[5]));
+            const float x[7] = {(float)(S[0] * o->cl_in), (float)(S[4] * o->cl_in),
+                                (float)(S[8] * o->cl_in), (float)(S[1] * o->cl_in),
+                                (float)(S[2] * o->cl_in), (float)(S[5] * o->cl_in),
+                                (float)(qn * o->cl_in)};
+            double z;
+            mlp_infer_bf16(o->mlp, x, 1, &z);
+            const double sp = z > 30.0 ? z : log1p(exp(z));
+            omega = (REAL)(1.0 / (1.0 / o->p.omega + o->cl_out * sp));
+        } else if (o->cs > 0) {
             /* compute_strain_tensor, LatticeStructure.hpp:105-122 */
             REAL S[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             REAL e[3];
@@ -308,4 +330,12 @@ void FN(oracle_set_populations)(struct FN(oracle) *o, const REAL *f)
     memcpy(o->fEven, f, sizeof(REAL) * o->N * o->Q);
     memcpy(o->fOdd, f, sizeof(REAL) * o->N * o->Q);
     o->fOut = (o->ts % 2) != 0 ? o->fEven : o->fOdd;
+}
+
+void FN(oracle_set_closure)(struct FN(oracle) *o, const float *params, double cl_in, double cl_out)
+{
+    o->closure = params != 0;
+    if (params) memcpy(o->mlp, params, sizeof(o->mlp));
+    o->cl_in = cl_in;
+    o->cl_out = cl_out;
 }

@@ -41,6 +41,19 @@ def lib():
                 n = {"get_macros": 4, "get_forces": 3, "get_populations": 1}[fn]
                 getattr(L, f"oracle_{fn}_{suf}").argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * n
             getattr(L, f"oracle_set_populations_{suf}").argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+            getattr(L, f"oracle_set_closure_{suf}").argtypes = [ctypes.c_void_p, ctypes.c_void_p,
+                                                               ctypes.c_double, ctypes.c_double]
+        L.mlp_forward_f64.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.mlp_loss_grad_f64.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_int, ctypes.c_void_p]
+        L.mlp_loss_grad_f64.restype = ctypes.c_double
+        L.mlp_adam_f64.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_double]
+        L.mlp_infer_bf16.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.mlp_round_bf16.argtypes = [ctypes.c_float]
+        L.mlp_round_bf16.restype = ctypes.c_float
+        L.mlp_nparam.restype = ctypes.c_int
         L.oracle_get_weights_f64.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
     return _lib
@@ -96,9 +109,58 @@ class Oracle:
             raise ValueError(f"expected {self.N * self.Q} populations, got {a.size}")
         getattr(lib(), f"oracle_set_populations_{self.suf}")(self._h, a.ctypes.data)
 
+    def set_closure(self, params, cl_in: float, cl_out: float) -> None:
+        if params is None:
+            getattr(lib(), f"oracle_set_closure_{self.suf}")(self._h, None, 0.0, 0.0)
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        getattr(lib(), f"oracle_set_closure_{self.suf}")(self._h, p.ctypes.data, cl_in, cl_out)
+
     def weights(self):
         if self.suf != "f64":
             raise ValueError("weights() reads the f64 table")
         w = np.empty(self.Q, np.float64)
         lib().oracle_get_weights_f64(self._h, w.ctypes.data)
         return w
+
+
+# ---- closure network (FP64 reference of the HIP training / inference) ----
+
+MLP_NPARAM = 1345
+
+
+def mlp_forward(params, x):
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 7)
+    y = np.empty(len(xx))
+    lib().mlp_forward_f64(p.ctypes.data, xx.ctypes.data, len(xx), y.ctypes.data)
+    return y
+
+
+def mlp_loss_grad(params, x, t):
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    xx = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 7)
+    tt = np.ascontiguousarray(t, dtype=np.float64)
+    g = np.empty(MLP_NPARAM)
+    loss = lib().mlp_loss_grad_f64(p.ctypes.data, xx.ctypes.data, tt.ctypes.data, len(xx), g.ctypes.data)
+    return loss, g
+
+
+def mlp_adam(params, grad, mom, vel, step, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+    for a in (params, mom, vel):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+    g = np.ascontiguousarray(grad, dtype=np.float64)
+    lib().mlp_adam_f64(params.ctypes.data, g.ctypes.data, mom.ctypes.data, vel.ctypes.data, step, lr,
+                       beta1, beta2, eps)
+
+
+def mlp_infer_bf16(params, x):
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 7)
+    z = np.empty(len(xx))
+    lib().mlp_infer_bf16(p.ctypes.data, xx.ctypes.data, len(xx), z.ctypes.data)
+    return z
+
+
+def round_bf16(v):
+    return np.array([lib().mlp_round_bf16(float(a)) for a in np.ravel(v)], dtype=np.float32)

@@ -219,7 +219,8 @@ def test_fixtures_catch_planted_oracle_mistake(plant, tmp_path):
     so = tmp_path / f"liboracle_plant{plant}.so"
     subprocess.run(["gcc", "-O2", "-std=c99", "-fPIC", "-shared", "-ffp-contract=off",
                     f"-DORACLE_PLANT={plant}", "-o", str(so),
-                    os.path.join(ROOT, "oracle", "lbm_oracle.c"), "-lm"], check=True)
+                    os.path.join(ROOT, "oracle", "lbm_oracle.c"),
+                    os.path.join(ROOT, "oracle", "mlp_oracle.c"), "-lm"], check=True)
     import pyoracle
     saved_lib, saved_path = pyoracle._lib, pyoracle.LIB_PATH
     pyoracle._lib, pyoracle.LIB_PATH = None, str(so)

@@ -25,7 +25,15 @@ struct Launch {
         using TU = Tune<ST>;
         switch (mode) {
         case MODE_STEP:
-            if (TU::kVector && a.vec) {
+            if (a.closure) {
+                if constexpr (ST == ST_F64) {
+                    return hipErrorInvalidValue;
+                } else {
+                    hipLaunchKernelGGL((k_lbm_closure<L, ST, TU::kBS, TU::kNTLoad>),
+                                       dim3((a.N + TU::kBS - 1) / TU::kBS), dim3(TU::kBS), 0, s, in,
+                                       out, nd, chunk_type, uyh, a);
+                }
+            } else if (TU::kVector && a.vec) {
                 constexpr uint32_t V = 16 / sizeof(st);
                 hipLaunchKernelGGL((k_lbm_vec<L, ST, 16>), dim3(blocks_for(a.N / V)), dim3(kBlock), 0,
                                    s, in, out, nd, chunk_type, uyh, a);
@@ -138,9 +146,16 @@ struct Launch {
     }
 #undef TLBM_BY_LATTICE
 
+    static hipError_t box_filter(const void *macro, double *out, const StepArgs &a, hipStream_t s) {
+        const uint32_t nc = (a.nx / 2) * (a.ny / 2) * (a.nz / 2);
+        hipLaunchKernelGGL((k_box_filter<T>), dim3(blocks_for(nc)), dim3(kBlock), 0, s,
+                           static_cast<const T *>(macro), out, a);
+        return hipGetLastError();
+    }
+
     static const LaunchOps &table() {
         static const LaunchOps t = {&lbm, &init_rest, &init_macros, &get_populations,
-                                    &set_populations, &forces, &reduce, int(sizeof(T))};
+                                    &set_populations, &forces, &reduce, &box_filter, int(sizeof(T))};
         return t;
     }
 };

@@ -26,6 +26,7 @@ struct LaunchOps {
                          const StepArgs &a, hipStream_t s);
     hipError_t (*reduce)(int lattice, const void *cur, const void *prev, double *partial,
                          int nparts, double *out, const StepArgs &a, hipStream_t s);
+    hipError_t (*box_filter)(const void *macro, double *out, const StepArgs &a, hipStream_t s);
     int macro_bytes;  // size of one macro value (the compute type)
 };
 

@@ -158,6 +158,40 @@ k_forces(const typename StorageT<ST>::st *__restrict__ fin, const uint8_t *__res
     F[2 * size_t(a.N) + g] = fz;
 }
 
+// 2x2x2 box filter of one step's moments (macro: ux, uy, uz, rho planes of
+// N values) onto the (nx/2, ny/2, nz/2) lattice: 10 planes of means,
+// rho, ux, uy, uz, uxux, uyuy, uzuz, uxuy, uxuz, uyuz.
+template <class T>
+__global__ void __launch_bounds__(kBlock)
+k_box_filter(const T *__restrict__ macro, double *__restrict__ out, StepArgs a) {
+    const uint32_t cx_n = a.nx / 2, cy_n = a.ny / 2, cz_n = a.nz / 2;
+    const uint32_t Nc = cx_n * cy_n * cz_n;
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= Nc) return;
+    const uint32_t cz = c / (cx_n * cy_n), cr = c - cz * cx_n * cy_n, cy = cr / cx_n, cx = cr - cy * cx_n;
+    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t dz = 0; dz < 2; ++dz)
+        for (uint32_t dy = 0; dy < 2; ++dy)
+            for (uint32_t dx = 0; dx < 2; ++dx) {
+                const uint32_t g = (2 * cx + dx) + a.nx * (2 * cy + dy) + a.nxny * (2 * cz + dz);
+                const double ux = double(macro[g]), uy = double(macro[size_t(a.N) + g]),
+                             uz = double(macro[2 * size_t(a.N) + g]),
+                             rho = double(macro[3 * size_t(a.N) + g]);
+                acc[0] += rho;
+                acc[1] += ux;
+                acc[2] += uy;
+                acc[3] += uz;
+                acc[4] += ux * ux;
+                acc[5] += uy * uy;
+                acc[6] += uz * uz;
+                acc[7] += ux * uy;
+                acc[8] += ux * uz;
+                acc[9] += uy * uz;
+            }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) out[size_t(k) * Nc + c] = acc[k] * 0.125;
+}
+
 // Per-workgroup FP64 partial sums over the state: sum f, sum f e_x,
 // sum f e_y, sum f e_z, sum (B_n - B_{n-1})^2, sum B_n^2 (kRed values).
 

@@ -35,6 +35,7 @@
 #include <stdint.h>
 
 #include "lattice.h"
+#include "mlp.h"
 
 namespace tlbm {
 
@@ -61,6 +62,13 @@ struct StepArgs {
     float c_in;        // 2*sum_{e_z=-1} w + sum_{e_z=0} w - (1 - u_bc)
     float c_out;       // 2*sum_{e_z=+1} w + sum_{e_z=0} w - rho_bc
     float drho_bc;     // rho_bc - 1
+    // learned closure (FP32/FP16 storage): tau = tau0 + cl_out * softplus(z),
+    // z = network(features * cl_in)
+    int closure;
+    float cl_in, cl_out, tau0;
+    const MlpPacked *mlp;
+    // per-node inlet velocity u_z(x, y) (nx*ny values), or null for u_bc
+    const float *inlet_uz;
 };
 
 // Offset of population s of node g in a state buffer (chunked layout).
@@ -184,14 +192,17 @@ __device__ inline T edot(int ex, int ey, int ez, T ux, T uy, T uz) {
     return d;
 }
 
-// The reference's node update on registers.  f[] holds the gathered
-// pre-collision populations on entry (deviations d = f - w when SHIFT) and
-// the post-collision ones on exit; rho/ux/uy/uz end as the values
-// write_data() would dump for this step.
+// The reference's node update on registers, in phases.  f[] holds the
+// gathered pre-collision populations on entry (deviations d = f - w when
+// SHIFT).  node_prepare computes the moments and applies the node type's
+// rule; rho/ux/uy/uz end as the values write_data() would dump for this
+// step.  With COLLIDE it also applies the type-5 rule and the inlet/outlet
+// reconstruction, fills feq and returns true for nodes that relax; a solid
+// node is bounced back in place (feq = f) and returns false.
 template <class L, class T, bool SHIFT, bool COLLIDE>
-__device__ inline void node_update(T (&f)[L::Q], int t, T &rho, T &ux, T &uy, T &uz,
-                                   const StepArgs &a, const T *__restrict__ uy_head,
-                                   const T (&w)[L::Q]) {
+__device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho, T &ux, T &uy,
+                                    T &uz, const StepArgs &a, const T *__restrict__ uy_head,
+                                    const T (&w)[L::Q], uint32_t g) {
     constexpr int Q = L::Q;
     T dlt = T(0);  // rho - 1 (SHIFT only)
     // compute_macroscopic_data, LatticeStructure.hpp:69-90
@@ -227,9 +238,12 @@ __device__ inline void node_update(T (&f)[L::Q], int t, T &rho, T &ux, T &uy, T 
 #pragma unroll
             for (int s = 0; s < Q; ++s) g[s] = f[L::bb[s]];
 #pragma unroll
-            for (int s = 0; s < Q; ++s) f[s] = g[s];
+            for (int s = 0; s < Q; ++s) {
+                f[s] = g[s];
+                feq[s] = g[s];
+            }
         }
-        return;
+        return false;
     }
     if (t == 2) {
         // set_inletW_bc_macro (D3Q19LatticeStructure.hpp:108-132)
@@ -238,12 +252,22 @@ __device__ inline void node_update(T (&f)[L::Q], int t, T &rho, T &ux, T &uy, T 
         for (int i = 0; i < L::n_unk; ++i) s_in += f[L::in_list[i]];
 #pragma unroll
         for (int i = 0; i < L::n_zero; ++i) sz += f[L::zero_list[i]];
-        ux = T(0); uy = T(0); uz = T(a.u_bc);
+        ux = T(0); uy = T(0);
+        T inv_1mu = T(a.inv_1mu), c_in = T(a.c_in);
+        if (a.inlet_uz) {
+            // a prescribed inlet profile: 1/(1-u) and the deviation constant per node
+            const T u = T(a.inlet_uz[g % a.nxny]);
+            uz = u;
+            inv_1mu = T(1) / (T(1) - u);
+            c_in = T(a.c_in) + (u - T(a.u_bc));
+        } else {
+            uz = T(a.u_bc);
+        }
         if constexpr (SHIFT) {
-            dlt = T(a.inv_1mu) * (T(a.c_in) + (T(2) * s_in + sz));
+            dlt = inv_1mu * (c_in + (T(2) * s_in + sz));
             rho = T(1) + dlt;
         } else {
-            rho = T(a.inv_1mu) * (T(2) * s_in + sz);
+            rho = inv_1mu * (T(2) * s_in + sz);
         }
     } else if (t == 3) {
         // set_outletE_bc_macro (D3Q19LatticeStructure.hpp:135-156); ux, uy kept
@@ -260,11 +284,11 @@ __device__ inline void node_update(T (&f)[L::Q], int t, T &rho, T &ux, T &uy, T 
             uz = T(-1) + T(a.inv_rho_bc) * (T(2) * s_out + sz);
         }
     }
-    if (!COLLIDE) return;
+    if (!COLLIDE) return true;
     if (t == 5) {
         // set_uz_bc (LatticeStructure.hpp:166-177); the y term reads uy of
         // node s of this step, as the reference's uy[spd] does
-        const T ubc = T(a.u_bc);
+        const T ubc = a.inlet_uz ? T(a.inlet_uz[g % a.nxny]) : T(a.u_bc);
 #pragma unroll
         for (int s = 1; s < Q; ++s) {
             T br = T(-0.0);
@@ -276,7 +300,6 @@ __device__ inline void node_update(T (&f)[L::Q], int t, T &rho, T &ux, T &uy, T 
     }
     // compute_equilibrium, LatticeStructure.hpp:92-103
     // (SHIFT: f_eq - w = w (delta + rho (cu + cu^2/2 - 3u^2/2)))
-    T feq[Q];
     {
         const T usq = ux * ux + uy * uy + uz * uz;
 #pragma unroll
@@ -297,32 +320,66 @@ __device__ inline void node_update(T (&f)[L::Q], int t, T &rho, T &ux, T &uy, T 
         for (int i = 0; i < L::n_unk; ++i)
             f[L::out_unk[i]] = feq[L::out_unk[i]] + f[L::out_par[i]] - feq[L::out_par[i]];
     }
+    return true;
+}
+
+// Non-equilibrium stress Pi = sum_s e e (f - feq) (compute_strain_tensor,
+// LatticeStructure.hpp:105-122): xx, yy, zz, xy, xz, yz.
+template <class L, class T>
+__device__ inline void node_stress(const T (&f)[L::Q], const T (&feq)[L::Q], T (&pi)[6]) {
+    T sxx = T(-0.0), syy = T(-0.0), szz = T(-0.0), sxy = T(-0.0), sxz = T(-0.0), syz = T(-0.0);
+#pragma unroll
+    for (int s = 0; s < L::Q; ++s) {
+        const T fn = f[s] - feq[s];
+        if (L::ex[s] != 0) sxx += fn;
+        if (L::ey[s] != 0) syy += fn;
+        if (L::ez[s] != 0) szz += fn;
+        if (L::ex[s] * L::ey[s] > 0) sxy += fn; else if (L::ex[s] * L::ey[s] < 0) sxy -= fn;
+        if (L::ex[s] * L::ez[s] > 0) sxz += fn; else if (L::ex[s] * L::ez[s] < 0) sxz -= fn;
+        if (L::ey[s] * L::ez[s] > 0) syz += fn; else if (L::ey[s] * L::ez[s] < 0) syz -= fn;
+    }
+    pi[0] = sxx; pi[1] = syy; pi[2] = szz; pi[3] = sxy; pi[4] = sxz; pi[5] = syz;
+}
+
+// |Pi| as apply_turbulence_model forms it (LatticeStructure.hpp:147).
+template <class T>
+__device__ inline T stress_norm(const T (&pi)[6]) {
+    return sqrt(pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +
+                T(2) * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5]));
+}
+
+// apply_turbulence_model, LatticeStructure.hpp:140-153.
+template <class T>
+__device__ inline T smagorinsky_omega(const T (&pi)[6], const StepArgs &a) {
+    T P = stress_norm<T>(pi);
+    P *= T(a.cs);
+    P = sqrt(P + T(a.nu2)) - T(a.nu);
+    const T nu_e = P / T(6);
+    return T(1) / (T(3) * (T(a.nu) + nu_e) + T(0.5));
+}
+
+// relax, LatticeStructure.hpp:155-163.
+template <class L, class T>
+__device__ inline void node_relax(T (&f)[L::Q], const T (&feq)[L::Q], T om) {
+#pragma unroll
+    for (int s = 0; s < L::Q; ++s) f[s] = f[s] - om * (f[s] - feq[s]);
+}
+
+// The whole reference update of one node (TLBM_Partition.cpp:742-807).
+template <class L, class T, bool SHIFT, bool COLLIDE>
+__device__ inline void node_update(T (&f)[L::Q], int t, T &rho, T &ux, T &uy, T &uz,
+                                   const StepArgs &a, const T *__restrict__ uy_head,
+                                   const T (&w)[L::Q], uint32_t g) {
+    T feq[L::Q];
+    if (!node_prepare<L, T, SHIFT, COLLIDE>(f, feq, t, rho, ux, uy, uz, a, uy_head, w, g)) return;
+    if (!COLLIDE) return;
     T om = T(a.omega);
     if (a.smag) {
-        // compute_strain_tensor + apply_turbulence_model,
-        // LatticeStructure.hpp:105-122, 140-153
-        T sxx = T(-0.0), syy = T(-0.0), szz = T(-0.0), sxy = T(-0.0), sxz = T(-0.0),
-          syz = T(-0.0);
-#pragma unroll
-        for (int s = 0; s < Q; ++s) {
-            const T fn = f[s] - feq[s];
-            if (L::ex[s] != 0) sxx += fn;
-            if (L::ey[s] != 0) syy += fn;
-            if (L::ez[s] != 0) szz += fn;
-            if (L::ex[s] * L::ey[s] > 0) sxy += fn; else if (L::ex[s] * L::ey[s] < 0) sxy -= fn;
-            if (L::ex[s] * L::ez[s] > 0) sxz += fn; else if (L::ex[s] * L::ez[s] < 0) sxz -= fn;
-            if (L::ey[s] * L::ez[s] > 0) syz += fn; else if (L::ey[s] * L::ez[s] < 0) syz -= fn;
-        }
-        T P = sqrt(sxx * sxx + syy * syy + szz * szz + T(2) * (sxy * sxy + sxz * sxz + syz * syz));
-        P *= T(a.cs);
-        P = sqrt(P + T(a.nu2)) - T(a.nu);
-        const T nu_e = P / T(6);
-        om = T(1) / (T(3) * (T(a.nu) + nu_e) + T(0.5));
+        T pi[6];
+        node_stress<L, T>(f, feq, pi);
+        om = smagorinsky_omega<T>(pi, a);
     }
-    if (a.dynamics == 1) {
-#pragma unroll
-        for (int s = 0; s < Q; ++s) f[s] = f[s] - om * (f[s] - feq[s]);
-    }
+    if (a.dynamics == 1) node_relax<L, T>(f, feq, om);
 }
 
 // One lattice update per thread.  MODE_STEP streams and collides
@@ -377,7 +434,7 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
     }
     T rho, ux, uy, uz;
     if (MODE == MODE_STEP) {
-        node_update<L, T, S::kShift, true>(f, t, rho, ux, uy, uz, a, uy_head, w);
+        node_update<L, T, S::kShift, true>(f, t, rho, ux, uy, uz, a, uy_head, w, g);
 #pragma unroll
         for (int s = 0; s < Q; ++s) {
             st *p = fout + fidx<Q>(s, g, a);
@@ -387,7 +444,7 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
                 *p = S::out(f[s], s, a);
         }
     } else {
-        node_update<L, T, S::kShift, false>(f, t, rho, ux, uy, uz, a, uy_head, w);
+        node_update<L, T, S::kShift, false>(f, t, rho, ux, uy, uz, a, uy_head, w, g);
         if (MODE == MODE_MACROS) {
             T *macro = static_cast<T *>(out);
             macro[g] = ux;
@@ -404,6 +461,60 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
             static_cast<T *>(out)[g] = uy;
         }
     }
+}
+
+// Step with the learned closure in place of Smagorinsky (FP32/FP16 storage).
+// One node per lane as k_lbm; every lane of a wavefront runs the network
+// together (lanes past the last node work on the last node and store
+// nothing).  A solid node's stress is zero and its result unused.
+template <class L, int ST, int BS, bool NTLOAD>
+__global__ void __launch_bounds__(BS)
+k_lbm_closure(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::st *__restrict__ fout,
+              const uint8_t *__restrict__ nd_type, const uint8_t *__restrict__ chunk_type,
+              const typename StorageT<ST>::T *__restrict__ uy_head, StepArgs a) {
+    using S = StorageT<ST>;
+    using T = typename S::T;
+    using st = typename S::st;
+    constexpr int Q = L::Q;
+    const uint32_t g0 = blockIdx.x * BS + threadIdx.x;
+    const bool live = g0 < a.N;
+    const uint32_t g = live ? g0 : a.N - 1;
+    const uint32_t z = g / a.nxny;
+    const uint32_t r = g - z * a.nxny;
+    const uint32_t y = r / a.nx;
+    const uint32_t x = r - y * a.nx;
+    Gather gat;
+    gat.init(x, y, z, a);
+    T f[Q];
+#pragma unroll
+    for (int s = 0; s < Q; ++s) {
+        const st *p = fin + fidx<Q>(s, gat.src(L::ex[s], L::ey[s], L::ez[s]), a);
+        f[s] = S::in(NTLOAD ? __builtin_nontemporal_load(p) : *p, s, a);
+    }
+    T w[Q];
+#pragma unroll
+    for (int s = 0; s < Q; ++s) w[s] = S::weight(s, a);
+    int t;
+    if (chunk_type) {
+        const uint8_t ct = chunk_type[(blockIdx.x * BS) >> a.cb];
+        t = (ct != 0xFF) ? ct : nd_type[g];
+    } else {
+        t = nd_type[g];
+    }
+    T feq[Q], rho, ux, uy, uz;
+    const bool relaxes = node_prepare<L, T, S::kShift, true>(f, feq, t, rho, ux, uy, uz, a, uy_head, w, g);
+    T pi[6];
+    node_stress<L, T>(f, feq, pi);
+    float feat[kMlpIn];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) feat[k] = float(pi[k]) * a.cl_in;
+    feat[6] = float(stress_norm<T>(pi)) * a.cl_in;
+    const float zz = mlp_wave(feat, a.mlp);
+    const T om = T(1) / (T(a.tau0) + T(a.cl_out) * T(softplus_f(zz)));
+    if (relaxes && a.dynamics == 1) node_relax<L, T>(f, feq, om);
+    if (!live) return;
+#pragma unroll
+    for (int s = 0; s < Q; ++s) __builtin_nontemporal_store(S::out(f[s], s, a), fout + fidx<Q>(s, g, a));
 }
 
 // ---------------------------------------------------------------------------
@@ -532,7 +643,7 @@ k_lbm_vec(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<S
         for (int s = 0; s < Q; ++s) f[s] = S::in(e[s][k], s, a);
         T rho, ux, uy, uz;
         const int tk = int((types >> (8 * k)) & 0xFFu);
-        node_update<L, T, S::kShift, true>(f, tk, rho, ux, uy, uz, a, uy_head, w);
+        node_update<L, T, S::kShift, true>(f, tk, rho, ux, uy, uz, a, uy_head, w, g0 + uint32_t(k));
 #pragma unroll
         for (int s = 0; s < Q; ++s) e[s][k] = S::out(f[s], s, a);
     }

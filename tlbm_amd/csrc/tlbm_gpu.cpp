@@ -12,8 +12,15 @@
 #include "launch.h"
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
+
+namespace tlbm {
+// Message for tlbm_gpu_last_error(), shared with the network entry points.
+void set_last_error(const std::string &msg) { g_err = msg; }
+}  // namespace tlbm
+
+namespace {
 
 int fail(int code, const std::string &msg) {
     g_err = msg;
@@ -99,6 +106,8 @@ struct tlbm_gpu {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t bytes = 0;
     std::vector<uint8_t> chunk_types;  // host copy, kept until the upload completes
+    tlbm::MlpPacked *mlp = nullptr;    // closure network, device copy
+    float *inlet = nullptr;            // nx*ny inlet u_z profile
     double *avg = nullptr;             // 4*N FP64 time-average sums
     bool avg_on = false;
     int64_t avg_steps = 0;
@@ -262,6 +271,8 @@ int tlbm_gpu_destroy(tlbm_gpu *h) {
     if (h->nd) (void)hipFree(h->nd);
     if (h->chunk_type) (void)hipFree(h->chunk_type);
     if (h->avg) (void)hipFree(h->avg);
+    if (h->mlp) (void)hipFree(h->mlp);
+    if (h->inlet) (void)hipFree(h->inlet);
     if (h->macro) (void)hipFree(h->macro);
     if (h->uy_head) (void)hipFree(h->uy_head);
     if (h->scratch) (void)hipFree(h->scratch);
@@ -464,6 +475,80 @@ int tlbm_gpu_get_time_average(tlbm_gpu *h, double *u, double *v, double *w, doub
                                     hipMemcpyDeviceToHost, h->stream));
     TLBM_HIP(hipStreamSynchronize(h->stream));
     if (nsteps) *nsteps = h->avg_steps;
+    return TLBM_OK;
+}
+
+int tlbm_gpu_set_closure(tlbm_gpu *h, const float *params, double feature_scale,
+                         double output_scale) {
+    if (int rc = check(h)) return rc;
+    if (!params) {
+        h->args.closure = 0;
+        return TLBM_OK;
+    }
+    if (h->p.storage == TLBM_STORAGE_F64)
+        return fail(TLBM_EINVAL, "the learned closure runs with FP32 or FP16 storage");
+    if (!(feature_scale > 0.0) || !(output_scale > 0.0))
+        return fail(TLBM_EINVAL, "closure scales must be positive");
+    for (int k = 0; k < tlbm::kMlpParams; ++k)
+        if (!std::isfinite(params[k])) return fail(TLBM_EINVAL, "closure parameters must be finite");
+    tlbm::MlpPacked packed;
+    tlbm::mlp_pack(params, &packed);
+    if (!h->mlp) {
+        TLBM_HIP(hipMalloc(&h->mlp, sizeof packed));
+        h->bytes += sizeof packed;
+    }
+    TLBM_HIP(hipMemcpyAsync(h->mlp, &packed, sizeof packed, hipMemcpyHostToDevice, h->stream));
+    TLBM_HIP(hipStreamSynchronize(h->stream));
+    h->args.closure = 1;
+    h->args.cl_in = float(feature_scale);
+    h->args.cl_out = float(output_scale);
+    h->args.tau0 = float(1.0 / h->p.omega);
+    h->args.mlp = h->mlp;
+    return TLBM_OK;
+}
+
+int tlbm_gpu_set_inlet_profile(tlbm_gpu *h, const double *uz) {
+    if (int rc = check(h)) return rc;
+    if (!uz) {
+        h->args.inlet_uz = nullptr;
+        return TLBM_OK;
+    }
+    const size_t n = size_t(h->args.nxny);
+    std::vector<float> v(n);
+    for (size_t i = 0; i < n; ++i) {
+        if (!(uz[i] < 1.0) || !std::isfinite(uz[i])) return fail(TLBM_EINVAL, "inlet u_z must be finite and < 1");
+        v[i] = float(uz[i]);
+    }
+    if (!h->inlet) {
+        TLBM_HIP(hipMalloc(&h->inlet, n * sizeof(float)));
+        h->bytes += n * sizeof(float);
+    }
+    TLBM_HIP(hipMemcpyAsync(h->inlet, v.data(), n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    TLBM_HIP(hipStreamSynchronize(h->stream));
+    h->args.inlet_uz = h->inlet;
+    return TLBM_OK;
+}
+
+int tlbm_gpu_box_filter(tlbm_gpu *h, double *out) {
+    if (int rc = check(h)) return rc;
+    if (!out) return fail(TLBM_EINVAL, "null output");
+    if ((h->args.nx | h->args.ny | h->args.nz) & 1u)
+        return fail(TLBM_EINVAL, "the box filter needs even nx, ny, nz");
+    if (h->steps_since_set < 1) return fail(TLBM_ESTATE, "the box filter reads the last step's moments");
+    if (!h->macro) {
+        const size_t b = size_t(4) * h->N * size_t(h->ops->macro_bytes);
+        TLBM_HIP(hipMalloc(&h->macro, b));
+        h->bytes += b;
+    }
+    hipError_t e = h->ops->lbm(h->Q, tlbm::MODE_MACROS, h->f[1 - h->cur], nullptr, h->nd, nullptr,
+                               h->macro, nullptr, h->args, h->stream);
+    if (e != hipSuccess) return fail(TLBM_EHIP, std::string("moments: ") + hipGetErrorString(e));
+    const size_t nc = size_t(h->args.nx / 2) * (h->args.ny / 2) * (h->args.nz / 2);
+    if (int rc = ensure_scratch(h, 10 * nc * sizeof(double))) return rc;
+    e = h->ops->box_filter(h->macro, h->scratch, h->args, h->stream);
+    if (e != hipSuccess) return fail(TLBM_EHIP, std::string("box filter: ") + hipGetErrorString(e));
+    TLBM_HIP(hipMemcpyAsync(out, h->scratch, 10 * nc * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    TLBM_HIP(hipStreamSynchronize(h->stream));
     return TLBM_OK;
 }
 

@@ -62,6 +62,22 @@ def load_library(path: str = LIB_PATH):
     L.tlbm_gpu_memory_bytes.argtypes = [_P, ctypes.POINTER(ctypes.c_uint64)]
     L.tlbm_gpu_time_average.argtypes = [_P, ctypes.c_int]
     L.tlbm_gpu_get_time_average.argtypes = [_P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_int64)]
+    L.tlbm_gpu_set_closure.argtypes = [_P, _P, ctypes.c_double, ctypes.c_double]
+    L.tlbm_gpu_box_filter.argtypes = [_P, _P]
+    L.tlbm_gpu_set_inlet_profile.argtypes = [_P, _P]
+    L.tlbm_mlp_trainer_create.argtypes = [_P, _P, ctypes.c_int64, _P, ctypes.c_uint64, ctypes.c_int,
+                                          ctypes.POINTER(_P)]
+    L.tlbm_mlp_trainer_step.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                        ctypes.POINTER(ctypes.c_double)]
+    L.tlbm_mlp_trainer_loss_grad.argtypes = [_P, ctypes.c_int64, ctypes.c_int64,
+                                             ctypes.POINTER(ctypes.c_double), _P]
+    L.tlbm_mlp_trainer_params.argtypes = [_P, _P]
+    L.tlbm_mlp_trainer_destroy.argtypes = [_P]
+    L.tlbm_mlp_infer.argtypes = [_P, _P, ctypes.c_int64, _P]
+    for name in ("tlbm_gpu_set_closure", "tlbm_gpu_box_filter", "tlbm_gpu_set_inlet_profile",
+                 "tlbm_mlp_trainer_create", "tlbm_mlp_trainer_step", "tlbm_mlp_trainer_loss_grad",
+                 "tlbm_mlp_trainer_params", "tlbm_mlp_trainer_destroy", "tlbm_mlp_infer"):
+        getattr(L, name).restype = ctypes.c_int
     L.tlbm_gpu_last_error.restype = ctypes.c_char_p
     L.tlbm_gpu_last_error.argtypes = []
     for name in ("create", "destroy", "step", "step_timed", "sync", "get_macros", "get_forces",
@@ -175,6 +191,33 @@ class Lattice:
                                                           ctypes.byref(n)))
         return dict(zip(("u", "v", "w", "rho"), out)), int(n.value)
 
+    def set_closure(self, params, feature_scale: float = 1.0, output_scale: float = 1.0) -> None:
+        """Learned closure in place of Smagorinsky (params None: off)."""
+        if params is None:
+            _check(load_library().tlbm_gpu_set_closure(self._h, None, 0.0, 0.0))
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        if p.size != MLP_NPARAM:
+            raise ValueError(f"the network has {MLP_NPARAM} parameters")
+        _check(load_library().tlbm_gpu_set_closure(self._h, p.ctypes.data, float(feature_scale),
+                                                   float(output_scale)))
+
+    def set_inlet_profile(self, uz) -> None:
+        if uz is None:
+            _check(load_library().tlbm_gpu_set_inlet_profile(self._h, None))
+            return
+        a = _f64(uz, self.shape[0] * self.shape[1])
+        _check(load_library().tlbm_gpu_set_inlet_profile(self._h, a.ctypes.data))
+
+    def box_filter(self) -> dict:
+        """2x2x2 box filter of the last step's moments (coarse global order)."""
+        nx, ny, nz = (d // 2 for d in self.shape)
+        out = np.empty(10 * nx * ny * nz, np.float64)
+        _check(load_library().tlbm_gpu_box_filter(self._h, out.ctypes.data))
+        out = out.reshape(10, -1)
+        names = ("rho", "ux", "uy", "uz", "uxux", "uyuy", "uzuz", "uxuy", "uxuz", "uyuz")
+        return dict(zip(names, out))
+
     def memory_bytes(self) -> int:
         b = ctypes.c_uint64()
         _check(load_library().tlbm_gpu_memory_bytes(self._h, ctypes.byref(b)))
@@ -186,3 +229,67 @@ def declared_symbols(header: str = HEADER_PATH) -> list:
     import re
     text = open(header).read()
     return sorted(set(re.findall(r"\b(tlbm_gpu_[a-z_0-9]+)\s*\(", text)))
+
+
+MLP_NPARAM = 1345
+
+
+def mlp_infer(params, x) -> np.ndarray:
+    """z of feature vectors through the collide kernel's wavefront inference."""
+    p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+    xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 7)
+    z = np.empty(len(xx), np.float32)
+    _check(load_library().tlbm_mlp_infer(p.ctypes.data, xx.ctypes.data, len(xx), z.ctypes.data))
+    return z
+
+
+class MlpTrainer:
+    """Adam training of the closure network on the GPU (HIP kernels)."""
+
+    def __init__(self, x, t, init, seed: int = 0, max_batch: int = 65536):
+        L = load_library()
+        self._x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 7)
+        self._t = np.ascontiguousarray(t, dtype=np.float32).ravel()
+        if len(self._x) != len(self._t):
+            raise ValueError("x and t differ in length")
+        p = np.ascontiguousarray(init, dtype=np.float32).ravel()
+        h = _P()
+        _check(L.tlbm_mlp_trainer_create(self._x.ctypes.data, self._t.ctypes.data, len(self._t),
+                                         p.ctypes.data, int(seed), int(max_batch), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load_library().tlbm_mlp_trainer_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, steps: int, batch: int, lr: float = 1e-3) -> float:
+        loss = ctypes.c_double()
+        _check(load_library().tlbm_mlp_trainer_step(self._h, int(steps), int(batch), float(lr),
+                                                    ctypes.byref(loss)))
+        return loss.value
+
+    def loss_grad(self, first: int, count: int):
+        loss = ctypes.c_double()
+        g = np.empty(MLP_NPARAM, np.float32)
+        _check(load_library().tlbm_mlp_trainer_loss_grad(self._h, int(first), int(count),
+                                                         ctypes.byref(loss), g.ctypes.data))
+        return loss.value, g
+
+    def params(self) -> np.ndarray:
+        p = np.empty(MLP_NPARAM, np.float32)
+        _check(load_library().tlbm_mlp_trainer_params(self._h, p.ctypes.data))
+        return p
+
+
+def mlp_init(seed: int = 0) -> np.ndarray:
+    """He-uniform weights, zero biases (b3 = -2: softplus(z) starts near 0.13)."""
+    rng = np.random.default_rng(seed)
+    W1 = rng.uniform(-1, 1, (32, 7)) * np.sqrt(6.0 / 7)
+    W2 = rng.uniform(-1, 1, (32, 32)) * np.sqrt(6.0 / 32)
+    w3 = rng.uniform(-1, 1, 32) * np.sqrt(6.0 / 32) * 0.1
+    p = np.concatenate([W1.ravel(), np.zeros(32), W2.ravel(), np.zeros(32), w3, [-2.0]])
+    return p.astype(np.float32)
