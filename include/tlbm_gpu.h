@@ -144,8 +144,14 @@ int tlbm_gpu_set_closure(tlbm_gpu *h, const float *params, double feature_scale,
                          double output_scale);
 
 /* 2x2x2 box filter of the last step's moments onto the (nx/2, ny/2, nz/2)
- * lattice (all even): 10 planes of doubles -- rho, ux, uy, uz, uxux, uyuy,
- * uzuz, uxuy, uxuz, uyuz (means of the 8 fine nodes), coarse global order. */
+ * lattice (all even): TLBM_FILTER_PLANES planes of doubles -- rho, ux, uy,
+ * uz, uxux, uyuy, uzuz, uxuy, uxuz, uyuz, then the viscous stress
+ * sigma = -(1 - omega/2) Pi_neq (xx, yy, zz, xy, xz, yz; omega of the node's
+ * BGK or Smagorinsky relaxation, zero on nodes that do not relax) -- means
+ * of the 8 fine nodes, coarse global order.  Not available while a closure
+ * network is set (TLBM_ESTATE).  No reference counterpart: it makes the
+ * closure's training data. */
+#define TLBM_FILTER_PLANES 16
 int tlbm_gpu_box_filter(tlbm_gpu *h, double *out);
 
 typedef struct tlbm_mlp_trainer tlbm_mlp_trainer;

@@ -12,10 +12,11 @@
 import numpy as np
 import pytest
 
+import lattice_tables
 import pyoracle
 from pyoracle import Oracle
 from tlbm_amd import geometry
-from tlbm_amd.gpu import MLP_NPARAM, Lattice, MlpTrainer, mlp_infer, mlp_init
+from tlbm_amd.gpu import FILTER_PLANES, MLP_NPARAM, Lattice, MlpTrainer, mlp_infer, mlp_init
 
 pytestmark = pytest.mark.gpu
 
@@ -135,10 +136,32 @@ def test_closure_run_follows_oracle(gpu_lib):
             assert np.linalg.norm(u - ur) / np.linalg.norm(ur) < 1e-4, n
 
 
+def _host_stress(A, e, w, omega, cs):
+    """Viscous stress -(1 - omega'/2) Pi_neq of pre-collision populations
+    A (N, Q) on fluid nodes, with the Smagorinsky omega' of
+    LatticeStructure.hpp:140-153 (numpy restatement)."""
+    rho = A.sum(axis=1)
+    u = (A @ e) / rho[:, None]
+    cu = 3.0 * (u @ e.T)
+    usq = (u * u).sum(axis=1)
+    feq = w[None, :] * rho[:, None] * (1.0 + cu + 0.5 * cu * cu - 1.5 * usq[:, None])
+    fn = A - feq
+    pairs = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))
+    pi = np.stack([fn @ (e[:, i] * e[:, j]).astype(np.float64) for i, j in pairs])
+    qn = np.sqrt(pi[0] ** 2 + pi[1] ** 2 + pi[2] ** 2 + 2.0 * (pi[3] ** 2 + pi[4] ** 2 + pi[5] ** 2))
+    nu0 = (1.0 / omega - 0.5) / 3.0
+    om = 1.0 / (3.0 * (nu0 + (np.sqrt(cs * qn + nu0 * nu0) - nu0) / 6.0) + 0.5)
+    return (0.5 * om - 1.0) * pi
+
+
 def test_box_filter_matches_host_means(gpu_lib):
     nx, ny, nz, nd = _channel()
-    with Lattice(19, nx, ny, nz, nd, storage="f64", omega=1.6, u_lbm=0.05) as lat:
-        lat.step(25)
+    omega, cs = 1.6, 0.1
+    e, _, w, _ = lattice_tables.tables(19)
+    with Lattice(19, nx, ny, nz, nd, storage="f64", omega=omega, u_lbm=0.05, cs=cs) as lat:
+        lat.step(24)
+        A = lat.populations()  # what the next step reads
+        lat.step(1)
         m = lat.macros()
         filt = lat.box_filter()
 
@@ -150,8 +173,23 @@ def test_box_filter_matches_host_means(gpu_lib):
             "uyuy": coarse(m["uy"] * m["uy"]), "uzuz": coarse(m["uz"] * m["uz"]),
             "uxuy": coarse(m["ux"] * m["uy"]), "uxuz": coarse(m["ux"] * m["uz"]),
             "uyuz": coarse(m["uy"] * m["uz"])}
+    assert set(filt) == set(FILTER_PLANES)
     for k, v in want.items():
         assert np.allclose(filt[k], v, rtol=1e-13, atol=1e-16), k
+    # stresses: cells of fluid nodes only (the restatement has no boundary rules)
+    sig = _host_stress(A, e, w, omega, cs)
+    all_fluid = coarse((nd == 0).astype(np.float64)) == 1.0
+    assert all_fluid.sum() > 50
+    scale = np.abs(sig[:, nd == 0]).max()
+    assert scale > 1e-6
+    for k, name in enumerate(("sxx", "syy", "szz", "sxy", "sxz", "syz")):
+        assert np.allclose(filt[name][all_fluid], coarse(sig[k])[all_fluid], rtol=1e-10,
+                           atol=1e-12 * scale), name
+    # solid nodes carry no stress
+    solid_cells = coarse((nd == 1).astype(np.float64)) == 1.0
+    assert solid_cells.any()
+    for name in ("sxx", "syy", "szz", "sxy", "sxz", "syz"):
+        assert np.all(filt[name][solid_cells] == 0.0)
 
 
 def test_inlet_profile(gpu_lib):

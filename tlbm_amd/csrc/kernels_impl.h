@@ -51,6 +51,10 @@ struct Launch {
             hipLaunchKernelGGL((k_lbm<L, ST, MODE_AVG>), dim3(blocks_for(a.N)), dim3(kBlock), 0,
                                s, in, out, nd, nullptr, mac, uyh, a);
             break;
+        case MODE_STRESS:
+            hipLaunchKernelGGL((k_lbm<L, ST, MODE_STRESS>), dim3(blocks_for(a.N)), dim3(kBlock), 0,
+                               s, in, out, nd, nullptr, mac, uyh, a);
+            break;
         case MODE_HEAD:
             hipLaunchKernelGGL((k_lbm<L, ST, MODE_HEAD>), dim3(1), dim3(64), 0, s, in, out, nd,
                                nullptr, mac, uyh, a);

@@ -158,9 +158,12 @@ k_forces(const typename StorageT<ST>::st *__restrict__ fin, const uint8_t *__res
     F[2 * size_t(a.N) + g] = fz;
 }
 
-// 2x2x2 box filter of one step's moments (macro: ux, uy, uz, rho planes of
-// N values) onto the (nx/2, ny/2, nz/2) lattice: 10 planes of means,
-// rho, ux, uy, uz, uxux, uyuy, uzuz, uxuy, uxuz, uyuz.
+// 2x2x2 box filter of one step's moments and stresses onto the
+// (nx/2, ny/2, nz/2) lattice.  macro: the 10 planes MODE_STRESS writes
+// (ux, uy, uz, rho, sigma xx, yy, zz, xy, xz, yz).  out: kFilterPlanes
+// planes of means, rho, ux, uy, uz, uxux, uyuy, uzuz, uxuy, uxuz, uyuz,
+// sxx, syy, szz, sxy, sxz, syz.  Sums in double, in a fixed order.
+
 template <class T>
 __global__ void __launch_bounds__(kBlock)
 k_box_filter(const T *__restrict__ macro, double *__restrict__ out, StepArgs a) {
@@ -169,7 +172,9 @@ k_box_filter(const T *__restrict__ macro, double *__restrict__ out, StepArgs a) 
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= Nc) return;
     const uint32_t cz = c / (cx_n * cy_n), cr = c - cz * cx_n * cy_n, cy = cr / cx_n, cx = cr - cy * cx_n;
-    double acc[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double acc[kFilterPlanes];
+#pragma unroll
+    for (int k = 0; k < kFilterPlanes; ++k) acc[k] = 0.0;
     for (uint32_t dz = 0; dz < 2; ++dz)
         for (uint32_t dy = 0; dy < 2; ++dy)
             for (uint32_t dx = 0; dx < 2; ++dx) {
@@ -187,9 +192,11 @@ k_box_filter(const T *__restrict__ macro, double *__restrict__ out, StepArgs a) 
                 acc[7] += ux * uy;
                 acc[8] += ux * uz;
                 acc[9] += uy * uz;
+#pragma unroll
+                for (int k = 0; k < 6; ++k) acc[10 + k] += double(macro[size_t(4 + k) * a.N + g]);
             }
 #pragma unroll
-    for (int k = 0; k < 10; ++k) out[size_t(k) * Nc + c] = acc[k] * 0.125;
+    for (int k = 0; k < kFilterPlanes; ++k) out[size_t(k) * Nc + c] = acc[k] * 0.125;
 }
 
 // Per-workgroup FP64 partial sums over the state: sum f, sum f e_x,

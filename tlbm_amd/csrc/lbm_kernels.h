@@ -40,10 +40,11 @@
 namespace tlbm {
 
 enum Storage : int { ST_F64 = 0, ST_F32 = 1, ST_F16 = 2 };
-enum Mode : int { MODE_STEP = 0, MODE_MACROS = 1, MODE_HEAD = 2, MODE_AVG = 3 };
+enum Mode : int { MODE_STEP = 0, MODE_MACROS = 1, MODE_HEAD = 2, MODE_AVG = 3, MODE_STRESS = 4 };
 
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
 constexpr int kRed = 6;      // values per FP64 reduction partial
+constexpr int kFilterPlanes = 16;  // box filter output planes
 
 // Launch arguments, passed by value (kernarg segment, scalar registers).
 struct StepArgs {
@@ -387,7 +388,10 @@ __device__ inline void node_update(T (&f)[L::Q], int t, T &rho, T &ux, T &uy, T 
 // (ux, uy, uz, rho planes of N compute-type values in `out`); MODE_AVG adds
 // those moments to four planes of N doubles in `out` (time average);
 // MODE_HEAD writes uy of nodes 0..Q-1 (the values the type-5 rule reads) to
-// `out`.
+// `out`; MODE_STRESS writes ux, uy, uz, rho and the viscous stress
+// sigma = -(1 - omega/2) Pi (xx, yy, zz, xy, xz, yz; omega of the BGK or
+// Smagorinsky relaxation, zero where the node does not relax) as 10 planes
+// of N compute-type values.
 // chunk_type (MODE_STEP, may be null): per chunk, the node type shared by
 // all its nodes, or 0xFF when they differ; a workgroup inside a uniform
 // chunk skips the per-node type load.
@@ -443,6 +447,22 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
             else
                 *p = S::out(f[s], s, a);
         }
+    } else if (MODE == MODE_STRESS) {
+        T feq[Q], pi[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+        T c = T(0);
+        if (node_prepare<L, T, S::kShift, true>(f, feq, t, rho, ux, uy, uz, a, uy_head, w, g) &&
+            a.dynamics == 1) {
+            node_stress<L, T>(f, feq, pi);
+            const T om = a.smag ? smagorinsky_omega<T>(pi, a) : T(a.omega);
+            c = T(0.5) * om - T(1);
+        }
+        T *macro = static_cast<T *>(out);
+        macro[g] = ux;
+        macro[size_t(a.N) + g] = uy;
+        macro[2 * size_t(a.N) + g] = uz;
+        macro[3 * size_t(a.N) + g] = rho;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) macro[size_t(4 + k) * a.N + g] = c * pi[k];
     } else {
         node_update<L, T, S::kShift, false>(f, t, rho, ux, uy, uz, a, uy_head, w, g);
         if (MODE == MODE_MACROS) {
@@ -510,8 +530,8 @@ k_lbm_closure(const typename StorageT<ST>::st *__restrict__ fin, typename Storag
     for (int k = 0; k < 6; ++k) feat[k] = float(pi[k]) * a.cl_in;
     feat[6] = float(stress_norm<T>(pi)) * a.cl_in;
     const float zz = mlp_wave(feat, a.mlp);
-    const T om = T(1) / (T(a.tau0) + T(a.cl_out) * T(softplus_f(zz)));
-    if (relaxes && a.dynamics == 1) node_relax<L, T>(f, feq, om);
+    const float om = __builtin_amdgcn_rcpf(a.tau0 + a.cl_out * softplus_f(zz));  // 1 ulp
+    if (relaxes && a.dynamics == 1) node_relax<L, T>(f, feq, T(om));
     if (!live) return;
 #pragma unroll
     for (int s = 0; s < Q; ++s) __builtin_nontemporal_store(S::out(f[s], s, a), fout + fidx<Q>(s, g, a));

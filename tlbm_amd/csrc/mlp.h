@@ -94,68 +94,73 @@ inline void mlp_pack(const float *p, MlpPacked *m) {
     m->b3 = b3[0];
 }
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Two floats to a bf16 pair, round to nearest even (v_cvt_pk_bf16_f32).
+__device__ inline uint32_t cvt_pk_bf16(float a, float b) {
+    const f32x2 v = {a, b};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+}
+
 __device__ inline bf16x4 pack4(float a, float b, float c, float d) {
-    bf16x4 r;
-    r[0] = short(f32_to_bf16(a));
-    r[1] = short(f32_to_bf16(b));
-    r[2] = short(f32_to_bf16(c));
-    r[3] = short(f32_to_bf16(d));
-    return r;
+    const u32x2 w = {cvt_pk_bf16(a, b), cvt_pk_bf16(c, d)};
+    return __builtin_bit_cast(bf16x4, w);
 }
 
 __device__ inline bf16x4 load_pair(const uint32_t (&w)[2]) {
-    bf16x4 r;
-    r[0] = short(w[0] & 0xffffu);
-    r[1] = short(w[0] >> 16);
-    r[2] = short(w[1] & 0xffffu);
-    r[3] = short(w[1] >> 16);
+    const u32x2 v = {w[0], w[1]};
+    return __builtin_bit_cast(bf16x4, v);
+}
+
+__device__ inline f32x16 load16(const float (&v)[16]) {
+    f32x16 r;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) r[k] = v[k];
     return r;
 }
 
 // z of this lane's node.  All 64 lanes must call it (idle lanes pass any
-// finite features and ignore the result).
+// finite features and ignore the result).  The two 32-node blocks run one
+// after the other so that one set of accumulators is live at a time; the
+// biases are the accumulators' initial values (bias first, as the sums of
+// the network's definition run).
 __device__ inline float mlp_wave(const float (&x)[kMlpIn], const MlpPacked *__restrict__ m) {
     const int l = int(threadIdx.x & 63u);
     const int h = l >> 5;
     float xo[kMlpIn];
 #pragma unroll
     for (int k = 0; k < kMlpIn; ++k) xo[k] = __shfl_xor(x[k], 32);
-    // layer-1 B operand: block A = nodes of lanes 0-31, block B = lanes 32-63;
+    // layer-1 B operand: block 0 = nodes of lanes 0-31, block 1 = lanes 32-63;
     // lane l supplies inputs 4h..4h+3 of node l%32 of each block
-    const bf16x4 xa = h ? pack4(xo[4], xo[5], xo[6], 0.f) : pack4(x[0], x[1], x[2], x[3]);
-    const bf16x4 xb = h ? pack4(x[4], x[5], x[6], 0.f) : pack4(xo[0], xo[1], xo[2], xo[3]);
+    const bf16x4 xb[2] = {h ? pack4(xo[4], xo[5], xo[6], 0.f) : pack4(x[0], x[1], x[2], x[3]),
+                          h ? pack4(x[4], x[5], x[6], 0.f) : pack4(xo[0], xo[1], xo[2], xo[3])};
     const bf16x4 a1 = load_pair(m->a1[l]);
-    const f32x16 zero = {0.f};
-    const f32x16 ca = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, xa, zero, 0, 0, 0);
-    const f32x16 cb = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, xb, zero, 0, 0, 0);
-    f32x16 da = zero, db = zero;
+    float part[2];
 #pragma unroll
-    for (int kb = 0; kb < 4; ++kb) {
-        // register 4kb+t holds unit 8kb+4h+t: the K rows of block kb
-        float ha[4], hb[4];
+    for (int blk = 0; blk < 2; ++blk) {
+        const f32x16 c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, xb[blk], load16(m->b1[h]), 0, 0, 0);
+        f32x16 d = load16(m->b2[h]);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const float bias = m->b1[h][4 * kb + t];
-            ha[t] = fmaxf(ca[4 * kb + t] + bias, 0.f);
-            hb[t] = fmaxf(cb[4 * kb + t] + bias, 0.f);
+        for (int kb = 0; kb < 4; ++kb) {
+            // register 4kb+t holds unit 8kb+4h+t: the K rows of block kb
+            const bf16x4 hk = pack4(fmaxf(c[4 * kb], 0.f), fmaxf(c[4 * kb + 1], 0.f),
+                                    fmaxf(c[4 * kb + 2], 0.f), fmaxf(c[4 * kb + 3], 0.f));
+            d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(load_pair(m->a2[kb][l]), hk, d, 0, 0, 0);
         }
-        const bf16x4 a2 = load_pair(m->a2[kb][l]);
-        da = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, pack4(ha[0], ha[1], ha[2], ha[3]), da, 0, 0, 0);
-        db = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a2, pack4(hb[0], hb[1], hb[2], hb[3]), db, 0, 0, 0);
-    }
-    float pa = 0.f, pb = 0.f;
+        float p = 0.f;
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const float w = m->w3[h][r];
-        const float bias = m->b2[h][r];
-        pa += w * fmaxf(da[r] + bias, 0.f);
-        pb += w * fmaxf(db[r] + bias, 0.f);
+        for (int r = 0; r < 16; ++r) p += m->w3[h][r] * fmaxf(d[r], 0.f);
+        part[blk] = p;
     }
-    pa += __shfl_xor(pa, 32);
-    pb += __shfl_xor(pb, 32);
+    const float pa = part[0] + __shfl_xor(part[0], 32);
+    const float pb = part[1] + __shfl_xor(part[1], 32);
     return (h ? pb : pa) + m->b3;
 }
 
-__device__ inline float softplus_f(float z) { return z > 20.f ? z : log1pf(__expf(z)); }
+// softplus with the hardware exp/log (a value below float resolution of
+// 1 + e^z is dropped: it moves the relaxation time by less than 1e-7).
+__device__ inline float softplus_f(float z) { return z > 20.f ? z : __logf(1.f + __expf(z)); }
 
 }  // namespace tlbm

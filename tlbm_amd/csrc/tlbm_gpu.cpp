@@ -95,6 +95,7 @@ struct tlbm_gpu {
     uint8_t *nd = nullptr;
     uint8_t *chunk_type = nullptr;  // per chunk: common node type or 0xFF
     void *macro = nullptr;    // 4*N compute-type values (dumps)
+    void *filt_in = nullptr;  // 10*N compute-type values (box filter input)
     void *uy_head = nullptr;  // Q compute-type values (type-5 rule)
     double *scratch = nullptr;  // staging for forces / populations / macros (lazy)
     size_t scratch_bytes = 0;
@@ -274,6 +275,7 @@ int tlbm_gpu_destroy(tlbm_gpu *h) {
     if (h->mlp) (void)hipFree(h->mlp);
     if (h->inlet) (void)hipFree(h->inlet);
     if (h->macro) (void)hipFree(h->macro);
+    if (h->filt_in) (void)hipFree(h->filt_in);
     if (h->uy_head) (void)hipFree(h->uy_head);
     if (h->scratch) (void)hipFree(h->scratch);
     if (h->partial) (void)hipFree(h->partial);
@@ -535,19 +537,25 @@ int tlbm_gpu_box_filter(tlbm_gpu *h, double *out) {
     if ((h->args.nx | h->args.ny | h->args.nz) & 1u)
         return fail(TLBM_EINVAL, "the box filter needs even nx, ny, nz");
     if (h->steps_since_set < 1) return fail(TLBM_ESTATE, "the box filter reads the last step's moments");
-    if (!h->macro) {
-        const size_t b = size_t(4) * h->N * size_t(h->ops->macro_bytes);
-        TLBM_HIP(hipMalloc(&h->macro, b));
+    if (h->args.closure)
+        return fail(TLBM_ESTATE, "the box filter's stress planes need the BGK or Smagorinsky relaxation");
+    if (!h->filt_in) {
+        const size_t b = size_t(10) * h->N * size_t(h->ops->macro_bytes);
+        TLBM_HIP(hipMalloc(&h->filt_in, b));
         h->bytes += b;
     }
-    hipError_t e = h->ops->lbm(h->Q, tlbm::MODE_MACROS, h->f[1 - h->cur], nullptr, h->nd, nullptr,
-                               h->macro, nullptr, h->args, h->stream);
-    if (e != hipSuccess) return fail(TLBM_EHIP, std::string("moments: ") + hipGetErrorString(e));
+    // the state the last step read is still intact; uy_head holds that
+    // step's type-5 values
+    hipError_t e = h->ops->lbm(h->Q, tlbm::MODE_STRESS, h->f[1 - h->cur], nullptr, h->nd, nullptr,
+                               h->filt_in, h->has_lid ? h->uy_head : nullptr, h->args, h->stream);
+    if (e != hipSuccess) return fail(TLBM_EHIP, std::string("stresses: ") + hipGetErrorString(e));
     const size_t nc = size_t(h->args.nx / 2) * (h->args.ny / 2) * (h->args.nz / 2);
-    if (int rc = ensure_scratch(h, 10 * nc * sizeof(double))) return rc;
-    e = h->ops->box_filter(h->macro, h->scratch, h->args, h->stream);
+    static_assert(tlbm::kFilterPlanes == TLBM_FILTER_PLANES, "box filter planes");
+    const size_t ob = size_t(tlbm::kFilterPlanes) * nc * sizeof(double);
+    if (int rc = ensure_scratch(h, ob)) return rc;
+    e = h->ops->box_filter(h->filt_in, h->scratch, h->args, h->stream);
     if (e != hipSuccess) return fail(TLBM_EHIP, std::string("box filter: ") + hipGetErrorString(e));
-    TLBM_HIP(hipMemcpyAsync(out, h->scratch, 10 * nc * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    TLBM_HIP(hipMemcpyAsync(out, h->scratch, ob, hipMemcpyDeviceToHost, h->stream));
     TLBM_HIP(hipStreamSynchronize(h->stream));
     return TLBM_OK;
 }

@@ -100,6 +100,9 @@ def _f64(a, n: int) -> np.ndarray:
     return a
 
 
+FILTER_PLANES = ("rho", "ux", "uy", "uz", "uxux", "uyuy", "uzuz", "uxuy", "uxuz", "uyuz",
+                 "sxx", "syy", "szz", "sxy", "sxz", "syz")
+
 class Lattice:
     """One lattice resident on the GPU: the reference's TLBM_Partition time
     step for a single partition, driven through the C-ABI."""
@@ -210,13 +213,12 @@ class Lattice:
         _check(load_library().tlbm_gpu_set_inlet_profile(self._h, a.ctypes.data))
 
     def box_filter(self) -> dict:
-        """2x2x2 box filter of the last step's moments (coarse global order)."""
+        """2x2x2 box filter of the last step's moments and viscous stresses
+        (coarse global order); keys in FILTER_PLANES."""
         nx, ny, nz = (d // 2 for d in self.shape)
-        out = np.empty(10 * nx * ny * nz, np.float64)
+        out = np.empty(len(FILTER_PLANES) * nx * ny * nz, np.float64)
         _check(load_library().tlbm_gpu_box_filter(self._h, out.ctypes.data))
-        out = out.reshape(10, -1)
-        names = ("rho", "ux", "uy", "uz", "uxux", "uyuy", "uzuz", "uxuy", "uxuz", "uyuz")
-        return dict(zip(names, out))
+        return dict(zip(FILTER_PLANES, out.reshape(len(FILTER_PLANES), -1)))
 
     def memory_bytes(self) -> int:
         b = ctypes.c_uint64()
@@ -264,6 +266,12 @@ class MlpTrainer:
         self._h = None
 
     def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
         self.close()
 
     def step(self, steps: int, batch: int, lr: float = 1e-3) -> float:
