@@ -78,3 +78,34 @@ def test_constant_network_closure_is_bgk_with_fixed_tau():
     a.step(20)
     b.step(20)
     assert np.array_equal(a.populations(), b.populations())
+
+
+def test_bf16_inference_matches_torch_emulation():
+    """The oracle's inference as the collide kernel runs it, against an
+    emulation built on torch's bfloat16 conversion: x, W1, W2, w3 and h1
+    rounded to bf16 (h1 from its float value), biases float, h2 and the
+    products in float64.  Dropping a rounding moves some outputs by far
+    more than the 1e-12 allowed here."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(5)
+    p = rng.normal(0, 0.4, MLP_NPARAM).astype(np.float32)
+    x = rng.normal(0, 1, (300, 7)).astype(np.float32)
+
+    def bf(a):
+        return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.bfloat16) \
+            .to(torch.float64).numpy()
+
+    W1 = p[:224].reshape(32, 7)
+    b1 = p[224:256].astype(np.float64)
+    W2 = p[256:1280].reshape(32, 32)
+    b2 = p[1280:1312].astype(np.float64)
+    w3 = p[1312:1344]
+    b3 = np.float64(p[1344])
+    pre1 = bf(x) @ bf(W1).T + b1
+    h1 = bf(np.maximum(pre1, 0.0).astype(np.float32))
+    want = np.maximum(h1 @ bf(W2).T + b2, 0.0) @ bf(w3) + b3
+    got = pyoracle.mlp_infer_bf16(p, x)
+    assert np.allclose(got, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    # the h1 rounding matters at this scale
+    h1_exact = np.maximum(pre1, 0.0)
+    assert np.abs(np.maximum(h1_exact @ bf(W2).T + b2, 0.0) @ bf(w3) + b3 - want).max() > 1e-6

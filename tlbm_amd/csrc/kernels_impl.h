@@ -29,8 +29,9 @@ struct Launch {
                 if constexpr (ST == ST_F64) {
                     return hipErrorInvalidValue;
                 } else {
-                    hipLaunchKernelGGL((k_lbm_closure<L, ST, TU::kBS, TU::kNTLoad>),
-                                       dim3((a.N + TU::kBS - 1) / TU::kBS), dim3(TU::kBS), 0, s, in,
+                    constexpr int CBS = kClosureBlock;
+                    hipLaunchKernelGGL((k_lbm_closure<L, ST, CBS, TU::kNTLoad>),
+                                       dim3((a.N + CBS - 1) / CBS), dim3(CBS), 0, s, in,
                                        out, nd, chunk_type, uyh, a);
                 }
             } else if (TU::kVector && a.vec) {

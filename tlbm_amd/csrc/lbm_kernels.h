@@ -36,6 +36,7 @@
 
 #include "lattice.h"
 #include "mlp.h"
+#include "udiv.h"
 
 namespace tlbm {
 
@@ -45,11 +46,16 @@ enum Mode : int { MODE_STEP = 0, MODE_MACROS = 1, MODE_HEAD = 2, MODE_AVG = 3, M
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
 constexpr int kRed = 6;      // values per FP64 reduction partial
 constexpr int kFilterPlanes = 16;  // box filter output planes
+constexpr int kClosureBlock = 512;  // workgroup of the closure collide
 
 // Launch arguments, passed by value (kernarg segment, scalar registers).
 struct StepArgs {
     uint32_t nx, ny, nz, nxny, N;
     uint32_t cb;       // log2 of the chunk size CH (nodes)
+    UDiv div_nx, div_nxny;  // g / nxny and r / nx without an integer divide
+    uint32_t kb;       // chunks a pull source lies at most before its node's chunk
+    int rel;           // 1: PullRel offsets fit 32 bits on this lattice
+    uint64_t fbytes;   // bytes of one state buffer
     int vec;           // 1: nx % V == 0, the step runs the 16-byte-per-lane kernel
     int dynamics;      // 1: BGK relaxation, other values: none (TLBM_Partition.cpp:795-807)
     int smag;          // cs > 0
@@ -82,6 +88,7 @@ __device__ inline size_t fidx(int s, uint32_t g, const StepArgs &a) {
 template <int ST> struct StorageT;
 template <> struct StorageT<ST_F64> {
     using st = double;
+    static constexpr int kCB = 12;  // 32 KiB chunks
     using T = double;
     static constexpr bool kShift = false;
     __device__ static inline T in(st r, int, const StepArgs &) { return r; }
@@ -96,6 +103,7 @@ template <> struct StorageT<ST_F64> {
 };
 template <> struct StorageT<ST_F32> {
     using st = float;
+    static constexpr int kCB = 13;  // 32 KiB chunks
     using T = float;
     static constexpr bool kShift = true;
     __device__ static inline T in(st r, int, const StepArgs &) { return r; }
@@ -114,6 +122,7 @@ template <> struct StorageT<ST_F32> {
 };
 template <> struct StorageT<ST_F16> {
     using st = _Float16;
+    static constexpr int kCB = 14;  // 32 KiB chunks
     using T = float;
     static constexpr bool kShift = true;
     __device__ static inline T in(st r, int s, const StepArgs &a) { return float(r) * a.h_unscale[s]; }
@@ -180,6 +189,117 @@ struct Gather {
     }
     __device__ inline uint32_t src(int ex, int ey, int ez) const {
         return xs[ex + 1] + ys[ey + 1] + zs[ez + 1];
+    }
+};
+
+// Buffer-resource access (raw buffer loads and stores: a scalar 128-bit
+// descriptor, a scalar offset per direction and one 32-bit lane offset).
+// kRsrcWord3 is the gfx9-family descriptor word (raw, untyped access);
+// kAuxNT the non-temporal cache bit of gfx942/gfx950 buffer instructions.
+constexpr int kRsrcWord3 = 0x00020000;
+constexpr int kAuxNT = 2;
+
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(uintptr_t base, uint64_t bytes) {
+    const int num = bytes >= 0xFFFFFFFFull ? -1 : int(uint32_t(bytes));
+    return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void *>(base), 0, num, kRsrcWord3);
+}
+
+template <class stt> struct RawBuf;
+template <> struct RawBuf<float> {
+    template <int AUX> __device__ static float load(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, int(v), int(s), AUX));
+    }
+    template <int AUX> __device__ static void store(float x, __amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, x), r, int(v), int(s), AUX);
+    }
+};
+template <> struct RawBuf<_Float16> {
+    template <int AUX> __device__ static _Float16 load(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        return __builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(r, int(v), int(s), AUX));
+    }
+    template <int AUX> __device__ static void store(_Float16 x, __amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, x), r, int(v), int(s), AUX);
+    }
+};
+template <> struct RawBuf<double> {
+    typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+    template <int AUX> __device__ static double load(__amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, int(v), int(s), AUX));
+    }
+    template <int AUX> __device__ static void store(double x, __amdgpu_buffer_rsrc_t r, uint32_t v, uint32_t s) {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x), r, int(v), int(s), AUX);
+    }
+};
+
+// Pull addressing of a workgroup whose BS nodes (first node g0, all in one
+// chunk c0, BS dividing CH) lie in planes 1 <= z <= nz - 2, so that no
+// source wraps in z.  A source node g' lies less than 2 nx ny nodes before
+// its node (the periodic y wrap of the plane behind), so with
+// kb = ceil(2 nx ny / CH) + 1 it lies after chunk c0 - kb, and its position relative to
+// chunk c0 - kb, r = g' - (c0 - kb) CH >= 0, gives the population's byte
+// offset from that chunk's start,
+//   (r + (r >> CB) (Q - 1) CH) * sizeof(st) + s CH sizeof(st),
+// a 32-bit lane offset (one add, one shift and one 24-bit multiply-add per
+// population) plus a scalar offset per direction on a buffer descriptor
+// based at chunk c0 - kb, where the general chunked index costs 64-bit
+// arithmetic per population.
+template <class L, int ST>
+struct PullRel {
+    using st = typename StorageT<ST>::st;
+    static constexpr uint32_t kSize = sizeof(st);
+    static constexpr int kLog = kSize == 8 ? 3 : (kSize == 4 ? 2 : 1);
+    static constexpr int CB = StorageT<ST>::kCB;
+    static constexpr uint32_t CH = 1u << CB;
+    static_assert((uint64_t(L::Q - 1) * CH * kSize) < (1u << 24), "24-bit multiplier");
+    __amdgpu_buffer_rsrc_t rsrc;    // based at chunk c0 - kb
+    uint32_t xs[3], ys[3], zs[3];   // byte offsets, summing to r * sizeof(st)
+    __device__ inline void init(const st *fin, uint32_t g0, uint32_t x, uint32_t y,
+                                uint32_t zplane, const StepArgs &a) {
+        const uint32_t c0 = g0 >> CB;
+        const intptr_t back = (intptr_t(c0) - intptr_t(a.kb)) * intptr_t(L::Q * CH * kSize);
+        rsrc = make_rsrc(uintptr_t(fin) + back, uint64_t(intptr_t(a.fbytes) - back));
+        xs[0] = ((x + 1 == a.nx) ? 0 : x + 1) << kLog;
+        xs[1] = x << kLog;
+        xs[2] = ((x == 0) ? a.nx - 1 : x - 1) << kLog;
+        ys[0] = (((y + 1 == a.ny) ? 0 : y + 1) * a.nx) << kLog;
+        ys[1] = (y * a.nx) << kLog;
+        ys[2] = (((y == 0) ? a.ny - 1 : y - 1) * a.nx) << kLog;
+        const uint32_t zr = zplane - (c0 - a.kb) * CH;  // modulo 2^32; the sums are exact
+        zs[0] = (zr + a.nxny) << kLog;
+        zs[1] = zr << kLog;
+        zs[2] = (zr - a.nxny) << kLog;
+    }
+    template <bool NT>
+    __device__ inline st load(int s, int ex, int ey, int ez) const {
+        const uint32_t rb = xs[ex + 1] + ys[ey + 1] + zs[ez + 1];
+        const uint32_t off = rb + __umul24(rb >> (CB + kLog), (L::Q - 1) * CH * kSize);
+        return RawBuf<st>::template load<NT ? kAuxNT : 0>(rsrc, off, uint32_t(s) * CH * kSize);
+    }
+};
+
+// Whether the workgroup of nodes [g0, g0 + BS) may use PullRel.
+template <int BS>
+__device__ inline bool pull_rel_ok(uint32_t g0, const StepArgs &a) {
+    return a.rel && g0 >= a.nxny && g0 + BS + a.nxny <= a.N;
+}
+
+// Aligned output of a workgroup inside chunk c0: a descriptor at the
+// chunk's start, a scalar offset per direction, one lane offset.
+template <class L, int ST>
+struct PushOut {
+    using st = typename StorageT<ST>::st;
+    static constexpr int CB = StorageT<ST>::kCB;
+    static constexpr uint32_t CH = 1u << CB;
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t off;
+    __device__ inline PushOut(st *fout, uint32_t c0, uint32_t g) {
+        rsrc = make_rsrc(uintptr_t(fout) + uintptr_t(c0) * L::Q * CH * sizeof(st),
+                         uint64_t(L::Q) * CH * sizeof(st));
+        off = (g & (CH - 1u)) * uint32_t(sizeof(st));
+    }
+    template <bool NT>
+    __device__ inline void store(int s, st v) const {
+        RawBuf<st>::template store<NT ? kAuxNT : 0>(v, rsrc, off, uint32_t(s) * CH * uint32_t(sizeof(st)));
     }
 };
 
@@ -414,17 +534,24 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
         g = blk * BS + threadIdx.x;
         if (g >= a.N) return;
     }
-    const uint32_t z = g / a.nxny;
+    const uint32_t z = udiv(g, a.div_nxny);
     const uint32_t r = g - z * a.nxny;
-    const uint32_t y = r / a.nx;
+    const uint32_t y = udiv(r, a.div_nx);
     const uint32_t x = r - y * a.nx;
-    Gather gat;
-    gat.init(x, y, z, a);
     T f[Q];
+    if (MODE == MODE_STEP && pull_rel_ok<BS>(blk * BS, a)) {
+        PullRel<L, ST> pr;
+        pr.init(fin, blk * BS, x, y, g - r, a);
 #pragma unroll
-    for (int s = 0; s < Q; ++s) {
-        const st *p = fin + fidx<Q>(s, gat.src(L::ex[s], L::ey[s], L::ez[s]), a);
-        f[s] = S::in(NTLOAD ? __builtin_nontemporal_load(p) : *p, s, a);
+        for (int s = 0; s < Q; ++s) f[s] = S::in(pr.template load<NTLOAD>(s, L::ex[s], L::ey[s], L::ez[s]), s, a);
+    } else {
+        Gather gat;
+        gat.init(x, y, z, a);
+#pragma unroll
+        for (int s = 0; s < Q; ++s) {
+            const st *p = fin + fidx<Q>(s, gat.src(L::ex[s], L::ey[s], L::ez[s]), a);
+            f[s] = S::in(NTLOAD ? __builtin_nontemporal_load(p) : *p, s, a);
+        }
     }
     T w[Q];
 #pragma unroll
@@ -439,14 +566,9 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
     T rho, ux, uy, uz;
     if (MODE == MODE_STEP) {
         node_update<L, T, S::kShift, true>(f, t, rho, ux, uy, uz, a, uy_head, w, g);
+        const PushOut<L, ST> po(fout, (blk * BS) >> StorageT<ST>::kCB, g);
 #pragma unroll
-        for (int s = 0; s < Q; ++s) {
-            st *p = fout + fidx<Q>(s, g, a);
-            if (NTSTORE)
-                __builtin_nontemporal_store(S::out(f[s], s, a), p);
-            else
-                *p = S::out(f[s], s, a);
-        }
+        for (int s = 0; s < Q; ++s) po.template store<NTSTORE>(s, S::out(f[s], s, a));
     } else if (MODE == MODE_STRESS) {
         T feq[Q], pi[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
         T c = T(0);
@@ -496,20 +618,34 @@ k_lbm_closure(const typename StorageT<ST>::st *__restrict__ fin, typename Storag
     using T = typename S::T;
     using st = typename S::st;
     constexpr int Q = L::Q;
+    // the network's packed weights, staged once per workgroup
+    __shared__ MlpPacked smlp;
+    {
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(a.mlp);
+        uint32_t *dst = reinterpret_cast<uint32_t *>(&smlp);
+        for (uint32_t i = threadIdx.x; i < sizeof(MlpPacked) / 4; i += BS) dst[i] = src[i];
+    }
     const uint32_t g0 = blockIdx.x * BS + threadIdx.x;
     const bool live = g0 < a.N;
     const uint32_t g = live ? g0 : a.N - 1;
-    const uint32_t z = g / a.nxny;
+    const uint32_t z = udiv(g, a.div_nxny);
     const uint32_t r = g - z * a.nxny;
-    const uint32_t y = r / a.nx;
+    const uint32_t y = udiv(r, a.div_nx);
     const uint32_t x = r - y * a.nx;
-    Gather gat;
-    gat.init(x, y, z, a);
     T f[Q];
+    if (pull_rel_ok<BS>(blockIdx.x * BS, a)) {
+        PullRel<L, ST> pr;
+        pr.init(fin, blockIdx.x * BS, x, y, g - r, a);
 #pragma unroll
-    for (int s = 0; s < Q; ++s) {
-        const st *p = fin + fidx<Q>(s, gat.src(L::ex[s], L::ey[s], L::ez[s]), a);
-        f[s] = S::in(NTLOAD ? __builtin_nontemporal_load(p) : *p, s, a);
+        for (int s = 0; s < Q; ++s) f[s] = S::in(pr.template load<NTLOAD>(s, L::ex[s], L::ey[s], L::ez[s]), s, a);
+    } else {
+        Gather gat;
+        gat.init(x, y, z, a);
+#pragma unroll
+        for (int s = 0; s < Q; ++s) {
+            const st *p = fin + fidx<Q>(s, gat.src(L::ex[s], L::ey[s], L::ez[s]), a);
+            f[s] = S::in(NTLOAD ? __builtin_nontemporal_load(p) : *p, s, a);
+        }
     }
     T w[Q];
 #pragma unroll
@@ -529,12 +665,14 @@ k_lbm_closure(const typename StorageT<ST>::st *__restrict__ fin, typename Storag
 #pragma unroll
     for (int k = 0; k < 6; ++k) feat[k] = float(pi[k]) * a.cl_in;
     feat[6] = float(stress_norm<T>(pi)) * a.cl_in;
-    const float zz = mlp_wave(feat, a.mlp);
+    __syncthreads();  // smlp complete (every lane of the workgroup gets here)
+    const float zz = mlp_wave(feat, &smlp);
     const float om = __builtin_amdgcn_rcpf(a.tau0 + a.cl_out * softplus_f(zz));  // 1 ulp
     if (relaxes && a.dynamics == 1) node_relax<L, T>(f, feq, T(om));
     if (!live) return;
+    const PushOut<L, ST> po(fout, (blockIdx.x * BS) >> StorageT<ST>::kCB, g);
 #pragma unroll
-    for (int s = 0; s < Q; ++s) __builtin_nontemporal_store(S::out(f[s], s, a), fout + fidx<Q>(s, g, a));
+    for (int s = 0; s < Q; ++s) po.template store<true>(s, S::out(f[s], s, a));
 }
 
 // ---------------------------------------------------------------------------

@@ -114,6 +114,25 @@ __device__ inline bf16x4 load_pair(const uint32_t (&w)[2]) {
     return __builtin_bit_cast(bf16x4, v);
 }
 
+// max(x, 0) as a signed-integer max of the bits: negative floats (sign bit
+// set) are negative integers; no NaN canonicalisation is inserted.
+__device__ inline float relu(float x) {
+    return __builtin_bit_cast(float, max(__builtin_bit_cast(int, x), 0));
+}
+
+// relu of both halves of a bf16 pair (v_pk_max_i16 with 0); rounding and
+// relu commute, so relu(round(x)) = round(relu(x)).
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+__device__ inline uint32_t relu_bf16x2(uint32_t v) {
+    const i16x2 z = {0, 0};
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(i16x2, v), z));
+}
+
+__device__ inline bf16x4 pack4_relu(float a, float b, float c, float d) {
+    const u32x2 w = {relu_bf16x2(cvt_pk_bf16(a, b)), relu_bf16x2(cvt_pk_bf16(c, d))};
+    return __builtin_bit_cast(bf16x4, w);
+}
+
 __device__ inline f32x16 load16(const float (&v)[16]) {
     f32x16 r;
 #pragma unroll
@@ -125,7 +144,9 @@ __device__ inline f32x16 load16(const float (&v)[16]) {
 // finite features and ignore the result).  The two 32-node blocks run one
 // after the other so that one set of accumulators is live at a time; the
 // biases are the accumulators' initial values (bias first, as the sums of
-// the network's definition run).
+// the network's definition run).  The output layer is FP32 on the vector
+// ALU: each lane sums w3 relu(h2) over its 16 units, the two lane halves
+// add up.
 __device__ inline float mlp_wave(const float (&x)[kMlpIn], const MlpPacked *__restrict__ m) {
     const int l = int(threadIdx.x & 63u);
     const int h = l >> 5;
@@ -145,13 +166,12 @@ __device__ inline float mlp_wave(const float (&x)[kMlpIn], const MlpPacked *__re
 #pragma unroll
         for (int kb = 0; kb < 4; ++kb) {
             // register 4kb+t holds unit 8kb+4h+t: the K rows of block kb
-            const bf16x4 hk = pack4(fmaxf(c[4 * kb], 0.f), fmaxf(c[4 * kb + 1], 0.f),
-                                    fmaxf(c[4 * kb + 2], 0.f), fmaxf(c[4 * kb + 3], 0.f));
+            const bf16x4 hk = pack4_relu(c[4 * kb], c[4 * kb + 1], c[4 * kb + 2], c[4 * kb + 3]);
             d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(load_pair(m->a2[kb][l]), hk, d, 0, 0, 0);
         }
         float p = 0.f;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) p += m->w3[h][r] * fmaxf(d[r], 0.f);
+        for (int r = 0; r < 16; ++r) p += m->w3[h][r] * relu(d[r]);
         part[blk] = p;
     }
     const float pa = part[0] + __shfl_xor(part[0], 32);

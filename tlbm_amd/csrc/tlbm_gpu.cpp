@@ -202,6 +202,16 @@ int tlbm_gpu_create(const tlbm_params *p, const uint8_t *nd_type, tlbm_gpu **out
     a.inv_1mu = 1. / (1. - p->u_lbm);
     a.inv_rho_bc = 1. / p->rho_lbm;
     a.cb = chunk_bits(storage_bytes(p->storage));
+    a.div_nx = tlbm::udiv_magic(a.nx);
+    a.div_nxny = tlbm::udiv_magic(a.nxny);
+    {
+        const uint64_t CH = uint64_t(1) << a.cb;
+        // a pull source lies less than 2 nx ny nodes before its node (the
+        // y wrap of the plane behind); then r < 2 kb CH
+        a.kb = uint32_t((2 * uint64_t(a.nxny) + CH - 1) / CH + 1);
+        a.rel = (2 * uint64_t(a.kb) + 2) * CH * uint64_t(Q) * storage_bytes(p->storage) <
+                (uint64_t(1) << 32);
+    }
     a.vec = (a.nx % uint32_t(16 / storage_bytes(p->storage))) == 0;
     switch (Q) {
     case 15: fill_weights<tlbm::D3Q15>(a, p->exact_weights != 0); break;
@@ -219,6 +229,7 @@ int tlbm_gpu_create(const tlbm_params *p, const uint8_t *nd_type, tlbm_gpu **out
         return cleanup_fail(fail(TLBM_EHIP, std::string("hipEventCreate: ") + hipGetErrorString(e)));
     const size_t nchunks = (size_t(N) + (size_t(1) << a.cb) - 1) >> a.cb;
     const size_t fbytes = (nchunks * Q << a.cb) * storage_bytes(p->storage);
+    a.fbytes = fbytes;
     for (int i = 0; i < 2; ++i) {
         if ((e = hipMalloc(&h->f[i], fbytes)) != hipSuccess)
             return cleanup_fail(fail(TLBM_ENOMEM, std::string("hipMalloc populations: ") +
