@@ -83,3 +83,17 @@ def test_rsa_pack_is_seeded_and_non_overlapping():
     np.fill_diagonal(d, np.inf)
     assert np.all(d >= r[:, None] + r[None, :])
     assert 4.0 / 3.0 * np.pi * (r ** 3).sum() >= 0.15 * 40 ** 3
+
+
+def test_rsa_pack_reaches_lattice_porosity():
+    n = 48
+    solid = np.zeros((n, n, n), bool)
+    solid[:, :, 0] = solid[:, :, -1] = solid[:, 0, :] = solid[:, -1, :] = True
+    walls = solid.copy()
+    c, r = geometry.rsa_spheres(n, n, n, 3, 5, porosity=0.8, seed=3, solid_mask=solid)
+    mask = geometry.sphere_pack_mask(n, n, n, c, r)
+    assert np.array_equal(solid, mask | walls)
+    porosity = 1.0 - solid.mean()
+    # stops with the first sphere that crosses the target
+    last = geometry.sphere_pack_mask(n, n, n, c[-1:], r[-1:]).sum()
+    assert 0.8 - last / n ** 3 <= porosity <= 0.8

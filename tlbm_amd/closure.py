@@ -576,6 +576,8 @@ def main(argv=None) -> int:
     ap.add_argument("--dump", default=None,
                     help="write the last --dump-snapshots filtered snapshots (npz) here")
     ap.add_argument("--dump-snapshots", type=int, default=8)
+    ap.add_argument("--save", default=None,
+                    help="write the trained network (params, feature and output scale) as JSON")
     a = ap.parse_args(argv)
     case = scaled_case(a.scale, snapshots=a.snapshots) if a.scale != 1.0 else \
         ClosureCase(snapshots=a.snapshots)
@@ -583,6 +585,12 @@ def main(argv=None) -> int:
     rep = run_pipeline(case, train_steps=tuple(a.train_steps), cost_shapes=shapes,
                        dump=a.dump, dump_snapshots=a.dump_snapshots)
     text = json.dumps(rep)
+    if a.save:
+        net = {k: rep[k] for k in ("params", "feature_scale", "output_scale", "case")}
+        net["calibration"] = rep["training"]["calibration"]
+        net["apriori"] = rep["training"]["apriori"]
+        with open(a.save, "w") as fh:
+            json.dump(net, fh)
     if a.out:
         with open(a.out, "w") as fh:
             fh.write(text)
@@ -591,6 +599,14 @@ def main(argv=None) -> int:
     summary["calibration"] = rep["training"]["calibration"]
     print(json.dumps(summary, indent=1))
     return 0
+
+
+def load_network(path: str):
+    """(params, feature_scale, output_scale) of a network saved by --save."""
+    with open(path) as fh:
+        net = json.load(fh)
+    return (np.asarray(net["params"], np.float32), float(net["feature_scale"]),
+            float(net["output_scale"]))
 
 
 if __name__ == "__main__":

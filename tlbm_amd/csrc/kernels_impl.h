@@ -34,10 +34,6 @@ struct Launch {
                                        dim3((a.N + CBS - 1) / CBS), dim3(CBS), 0, s, in,
                                        out, nd, chunk_type, uyh, a);
                 }
-            } else if (TU::kVector && a.vec) {
-                constexpr uint32_t V = 16 / sizeof(st);
-                hipLaunchKernelGGL((k_lbm_vec<L, ST, 16>), dim3(blocks_for(a.N / V)), dim3(kBlock), 0,
-                                   s, in, out, nd, chunk_type, uyh, a);
             } else {
                 hipLaunchKernelGGL((k_lbm<L, ST, MODE_STEP, false, TU::kNTLoad, true, TU::kBS>),
                                    dim3((a.N + TU::kBS - 1) / TU::kBS), dim3(TU::kBS), 0, s, in, out,

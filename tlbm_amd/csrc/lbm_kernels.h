@@ -56,7 +56,6 @@ struct StepArgs {
     uint32_t kb;       // chunks a pull source lies at most before its node's chunk
     int rel;           // 1: PullRel offsets fit 32 bits on this lattice
     uint64_t fbytes;   // bytes of one state buffer
-    int vec;           // 1: nx % V == 0, the step runs the 16-byte-per-lane kernel
     int dynamics;      // 1: BGK relaxation, other values: none (TLBM_Partition.cpp:795-807)
     int smag;          // cs > 0
     double cs, omega, nu, nu2, u_bc, rho_bc, inv_1mu, inv_rho_bc;
@@ -143,22 +142,21 @@ template <> struct StorageT<ST_F16> {
 };
 
 // Launch choices per storage, measured on MI355X with the D3Q19
-// 128x128x512 lattice: FP64 runs the 16-byte vector kernel (two nodes per
-// lane); FP32 and FP16 run one node per lane in 512-thread workgroups, FP16
-// with non-temporal loads.  Every kernel stores non-temporally.
+// 128x128x512 lattice: one node per lane everywhere, FP64 in 1024-thread
+// workgroups with non-temporal loads (ahead of a two-nodes-per-lane 16-byte
+// variant: 20.9 against 19.8 GLUPS, and 13.5 against 12.0 for D3Q27),
+// FP32 and FP16 in 512-thread workgroups, FP16 with non-temporal loads.
+// Every kernel stores non-temporally.
 template <int ST> struct Tune;
 template <> struct Tune<ST_F64> {
-    static constexpr bool kVector = true;
     static constexpr int kBS = 1024;
     static constexpr bool kNTLoad = true;
 };
 template <> struct Tune<ST_F32> {
-    static constexpr bool kVector = false;
     static constexpr int kBS = 512;
     static constexpr bool kNTLoad = false;
 };
 template <> struct Tune<ST_F16> {
-    static constexpr bool kVector = false;
     static constexpr int kBS = 512;
     static constexpr bool kNTLoad = true;
 };
@@ -673,144 +671,6 @@ k_lbm_closure(const typename StorageT<ST>::st *__restrict__ fin, typename Storag
     const PushOut<L, ST> po(fout, (blockIdx.x * BS) >> StorageT<ST>::kCB, g);
 #pragma unroll
     for (int s = 0; s < Q; ++s) po.template store<true>(s, S::out(f[s], s, a));
-}
-
-// ---------------------------------------------------------------------------
-// Vector path: each lane owns V = VB/sizeof(st) consecutive x-nodes, so every
-// population plane moves as one VB-byte load and one VB-byte store per lane
-// (non-temporal).  The pull gather shifts a plane by -e_x along x; the one
-// element a lane is missing sits in the neighbouring lane's vector and
-// arrives by a DPP wavefront shift (wave_shr:1 / wave_shl:1: lane i reads
-// lane i-1 / i+1).  Lanes at a wavefront edge or at the periodic x wrap load
-// that element instead.  Requires nx % V == 0 (so rows, chunks and vectors
-// align); whole waves stay active through the shifts.
-
-template <int VB> struct RawVec;
-template <> struct RawVec<8> { typedef uint32_t type __attribute__((ext_vector_type(2))); };
-template <> struct RawVec<16> { typedef uint32_t type __attribute__((ext_vector_type(4))); };
-
-__device__ inline uint32_t dpp_from_prev_lane(uint32_t v) {
-    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x138, 0xF, 0xF, false));
-}
-__device__ inline uint32_t dpp_from_next_lane(uint32_t v) {
-    return uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x130, 0xF, 0xF, false));
-}
-
-// Value of `v` in the lane before (PREV) or after this one.
-template <bool PREV, class st>
-__device__ inline st lane_shift(st v) {
-    if constexpr (sizeof(st) == 8) {
-        uint64_t b;
-        __builtin_memcpy(&b, &v, 8);
-        const uint32_t lo = PREV ? dpp_from_prev_lane(uint32_t(b)) : dpp_from_next_lane(uint32_t(b));
-        const uint32_t hi = PREV ? dpp_from_prev_lane(uint32_t(b >> 32)) : dpp_from_next_lane(uint32_t(b >> 32));
-        b = (uint64_t(hi) << 32) | lo;
-        st r;
-        __builtin_memcpy(&r, &b, 8);
-        return r;
-    } else if constexpr (sizeof(st) == 4) {
-        uint32_t b;
-        __builtin_memcpy(&b, &v, 4);
-        b = PREV ? dpp_from_prev_lane(b) : dpp_from_next_lane(b);
-        st r;
-        __builtin_memcpy(&r, &b, 4);
-        return r;
-    } else {
-        uint16_t h;
-        __builtin_memcpy(&h, &v, 2);
-        const uint32_t b = PREV ? dpp_from_prev_lane(uint32_t(h)) : dpp_from_next_lane(uint32_t(h));
-        h = uint16_t(b);
-        st r;
-        __builtin_memcpy(&r, &h, 2);
-        return r;
-    }
-}
-
-template <int V> struct NodeTypesT;
-template <> struct NodeTypesT<2> { using type = uint16_t; };
-template <> struct NodeTypesT<4> { using type = uint32_t; };
-template <> struct NodeTypesT<8> { using type = uint64_t; };
-
-template <class L, int ST, int VB, int MINW = 1>
-__global__ void __launch_bounds__(kBlock, MINW)
-k_lbm_vec(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::st *__restrict__ fout,
-          const uint8_t *__restrict__ nd_type, const uint8_t *__restrict__ chunk_type,
-          const typename StorageT<ST>::T *__restrict__ uy_head, StepArgs a) {
-    using S = StorageT<ST>;
-    using T = typename S::T;
-    using st = typename S::st;
-    using raw_t = typename RawVec<VB>::type;
-    constexpr int Q = L::Q;
-    constexpr int V = VB / int(sizeof(st));
-    static_assert(V >= 2, "a vector lane holds at least two nodes");
-    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-    const bool active = t < a.N / V;
-    const uint32_t g0 = active ? t * V : 0;  // idle lanes still feed the lane shifts
-    const uint32_t z = g0 / a.nxny;
-    const uint32_t r = g0 - z * a.nxny;
-    const uint32_t y = r / a.nx;
-    const uint32_t x0 = r - y * a.nx;
-    Gather gat;
-    gat.init(x0, y, z, a);
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t xe = x0 + V - 1;
-    const uint32_t x_prev = gat.xs[2];                     // x0 - 1, periodic
-    const uint32_t x_next = (xe + 1 == a.nx) ? 0 : xe + 1;  // xe + 1, periodic
-    const bool own_prev = lane == 0 || x0 == 0;
-    const bool own_next = lane == 63 || xe + 1 == a.nx;
-    st e[Q][V];
-#pragma unroll
-    for (int s = 0; s < Q; ++s) {
-        const int ex = L::ex[s];
-        const uint32_t row = gat.ys[L::ey[s] + 1] + gat.zs[L::ez[s] + 1];
-        const raw_t raw = __builtin_nontemporal_load(
-            reinterpret_cast<const raw_t *>(fin + fidx<Q>(s, row + x0, a)));
-        st v[V];
-        __builtin_memcpy(v, &raw, VB);
-        if (ex == 0) {
-#pragma unroll
-            for (int k = 0; k < V; ++k) e[s][k] = v[k];
-        } else if (ex > 0) {  // elements x0-1 .. xe-1
-            st p = lane_shift<true>(v[V - 1]);
-            if (own_prev) p = fin[fidx<Q>(s, row + x_prev, a)];
-            e[s][0] = p;
-#pragma unroll
-            for (int k = 1; k < V; ++k) e[s][k] = v[k - 1];
-        } else {  // elements x0+1 .. xe+1
-            st n = lane_shift<false>(v[0]);
-            if (own_next) n = fin[fidx<Q>(s, row + x_next, a)];
-#pragma unroll
-            for (int k = 0; k < V - 1; ++k) e[s][k] = v[k + 1];
-            e[s][V - 1] = n;
-        }
-    }
-    if (!active) return;
-    typename NodeTypesT<V>::type types;
-    const uint8_t ct = chunk_type ? chunk_type[(blockIdx.x * kBlock * V) >> a.cb] : uint8_t(0xFF);
-    if (ct != 0xFF)
-        types = typename NodeTypesT<V>::type(ct) * (~typename NodeTypesT<V>::type(0) / 0xFFu);
-    else
-        __builtin_memcpy(&types, nd_type + g0, sizeof(types));
-    T w[Q];
-#pragma unroll
-    for (int s = 0; s < Q; ++s) w[s] = S::weight(s, a);
-#pragma unroll
-    for (int k = 0; k < V; ++k) {
-        T f[Q];
-#pragma unroll
-        for (int s = 0; s < Q; ++s) f[s] = S::in(e[s][k], s, a);
-        T rho, ux, uy, uz;
-        const int tk = int((types >> (8 * k)) & 0xFFu);
-        node_update<L, T, S::kShift, true>(f, tk, rho, ux, uy, uz, a, uy_head, w, g0 + uint32_t(k));
-#pragma unroll
-        for (int s = 0; s < Q; ++s) e[s][k] = S::out(f[s], s, a);
-    }
-#pragma unroll
-    for (int s = 0; s < Q; ++s) {
-        raw_t o;
-        __builtin_memcpy(&o, e[s], VB);
-        __builtin_nontemporal_store(o, reinterpret_cast<raw_t *>(fout + fidx<Q>(s, g0, a)));
-    }
 }
 
 }  // namespace tlbm

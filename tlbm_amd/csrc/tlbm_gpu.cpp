@@ -212,7 +212,6 @@ int tlbm_gpu_create(const tlbm_params *p, const uint8_t *nd_type, tlbm_gpu **out
         a.rel = (2 * uint64_t(a.kb) + 2) * CH * uint64_t(Q) * storage_bytes(p->storage) <
                 (uint64_t(1) << 32);
     }
-    a.vec = (a.nx % uint32_t(16 / storage_bytes(p->storage))) == 0;
     switch (Q) {
     case 15: fill_weights<tlbm::D3Q15>(a, p->exact_weights != 0); break;
     case 19: fill_weights<tlbm::D3Q19>(a, p->exact_weights != 0); break;

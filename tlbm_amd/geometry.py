@@ -99,14 +99,21 @@ def periodic_box(nx: int, ny: int, nz: int, obstacles=()) -> np.ndarray:
 
 
 def rsa_spheres(nx: int, ny: int, nz: int, r_min: float, r_max: float, porosity: float,
-                seed: int, max_attempts: int = 2_000_000, walls=True):
+                seed: int, max_attempts: int = 2_000_000, walls=True, solid_mask=None,
+                z_margin: float = 0.0):
     """Seeded random sequential addition of non-overlapping spheres.
 
     Radii are uniform in [r_min, r_max], centres uniform in the box; a
     candidate is rejected if it overlaps an accepted sphere.  Spheres are
-    added until the solid fraction of the obstacle mask reaches
-    1 - porosity or max_attempts candidates have been drawn.  A cell list
-    keyed on the largest diameter keeps the overlap test local.
+    added until the solid fraction reaches 1 - porosity or max_attempts
+    candidates have been drawn.  The solid fraction is the spheres' volume
+    4/3 pi r^3, or, with solid_mask (a (nz, ny, nx) bool array of solid
+    nodes to start from, e.g. the walls; updated in place), the nodes of
+    that mask together with the nodes inside the spheres (d^2 < r^2), so
+    that the lattice's own porosity is the one reached.  z_margin > 0 keeps
+    every sphere that many nodes clear of the planes z = 0 and z = nz - 1
+    (centre z uniform in [z_margin + r, nz - 1 - z_margin - r]).  A cell
+    list keyed on the largest diameter keeps the overlap test local.
     Returns (centres, radii) as float64 arrays.
     """
     rng = np.random.default_rng(seed)
@@ -115,12 +122,15 @@ def rsa_spheres(nx: int, ny: int, nz: int, r_min: float, r_max: float, porosity:
     cells: dict = {}
     centres, radii = [], []
     target = (1.0 - porosity) * nx * ny * nz
-    solid = 0.0
+    solid = 0.0 if solid_mask is None else float(solid_mask.sum())
     attempts = 0
     while solid < target and attempts < max_attempts:
         attempts += 1
         r = rng.uniform(r_min, r_max)
-        c = rng.uniform((0.0, 0.0, 0.0), (nx, ny, nz))
+        if z_margin > 0:
+            c = rng.uniform((0.0, 0.0, z_margin + r), (nx, ny, nz - 1 - z_margin - r))
+        else:
+            c = rng.uniform((0.0, 0.0, 0.0), (nx, ny, nz))
         ci = (min(int(c[0] / cell), ncx - 1), min(int(c[1] / cell), ncy - 1),
               min(int(c[2] / cell), ncz - 1))
         ok = True
@@ -143,18 +153,32 @@ def rsa_spheres(nx: int, ny: int, nz: int, r_min: float, r_max: float, porosity:
         cells.setdefault(ci, []).append(len(centres))
         centres.append(c)
         radii.append(r)
-        solid += 4.0 / 3.0 * np.pi * r ** 3
+        if solid_mask is None:
+            solid += 4.0 / 3.0 * np.pi * r ** 3
+        else:
+            box, inside = _sphere_box(nx, ny, nz, c, r)
+            sub = solid_mask[box]
+            solid += float(np.count_nonzero(inside & ~sub))
+            sub |= inside
     return np.array(centres, dtype=np.float64).reshape(-1, 3), np.array(radii, dtype=np.float64)
+
+
+def _sphere_box(nx, ny, nz, c, r):
+    """(bounding-box slices, nodes strictly inside the sphere there)."""
+    cx, cy, cz = c
+    x0, x1 = max(0, int(np.floor(cx - r))), min(nx, int(np.ceil(cx + r)) + 1)
+    y0, y1 = max(0, int(np.floor(cy - r))), min(ny, int(np.ceil(cy + r)) + 1)
+    z0, z1 = max(0, int(np.floor(cz - r))), min(nz, int(np.ceil(cz + r)) + 1)
+    xx = (np.arange(x0, x1) - cx)[None, None, :] ** 2
+    yy = (np.arange(y0, y1) - cy)[None, :, None] ** 2
+    zz = (np.arange(z0, z1) - cz)[:, None, None] ** 2
+    return (slice(z0, z1), slice(y0, y1), slice(x0, x1)), xx + yy + zz < r * r
 
 
 def sphere_pack_mask(nx, ny, nz, centres, radii):
     """Obstacle mask for a sphere pack, evaluated per sphere bounding box."""
     mask = np.zeros((nz, ny, nx), dtype=bool)
-    for (cx, cy, cz), r in zip(centres, radii):
-        x0, x1 = max(0, int(np.floor(cx - r))), min(nx, int(np.ceil(cx + r)) + 1)
-        y0, y1 = max(0, int(np.floor(cy - r))), min(ny, int(np.ceil(cy + r)) + 1)
-        z0, z1 = max(0, int(np.floor(cz - r))), min(nz, int(np.ceil(cz + r)) + 1)
-        zz, yy, xx = np.meshgrid(np.arange(z0, z1), np.arange(y0, y1), np.arange(x0, x1),
-                                 indexing="ij")
-        mask[z0:z1, y0:y1, x0:x1] |= (xx - cx) ** 2 + (yy - cy) ** 2 + (zz - cz) ** 2 < r * r
+    for c, r in zip(centres, radii):
+        box, inside = _sphere_box(nx, ny, nz, c, r)
+        mask[box] |= inside
     return mask

@@ -1,0 +1,281 @@
+"""Runners of the measured workloads besides the stream-collide benchmark.
+
+* porous: D3Q27 Smagorinsky (cs 0.1, the reference's linear-cs form,
+  include/LatticeStructure.hpp:140-153) porous channel, 384^3, seeded
+  random sequential addition of non-overlapping spheres (radius uniform in
+  [6, 14], kept 4 nodes clear of the inlet and outlet planes) to porosity
+  0.65, bounce-back side walls, Zou-He inlet u_z 0.05 and outlet rho 1,
+  tau_0 0.505, 5000 steps; FP32 (deviation) storage against the FP64 run of
+  the same lattice.  Started impulsively this flow leaves the lattice
+  Boltzmann regime within ~100 steps and diverges (FP64 and FP32 alike; the
+  oracle, which follows the reference's rules, does the same on smaller
+  packs); the inlet velocity is therefore raised over the first 3000 steps
+  (--ramp 0 runs the impulsive start).
+* spheroid: D3Q19 1024x512x512 flow past a prolate spheroid (semi-axes 96
+  along z, 32 along x and y, centred at (512, 256, 256)), bounce-back side
+  walls, inlet u_z 0.05, outlet rho 1, tau 0.51, 10000 steps; FP16
+  deviation storage with the closure network in the collide kernel against
+  an FP32 Smagorinsky run; and on the 2x-downsampled copy (512x256x256,
+  semi-axes 48/16, tau 0.505 -- acoustic scaling keeps the lattice velocity
+  and halves the lattice viscosity -- 5000 steps) FP64, FP32 and FP16 runs,
+  the FP64 Smagorinsky run as the reference.
+
+Every run is timed with HIP events around batches of steps (kernel time on
+the lattice's stream), so MLUPS = nodes x steps / time, counting every node
+as the reference does (tLBM.cpp:71-72), and HBM GB/s = MLUPS x 2 Q bytes per
+value (each population read once and written once).  Errors: relative L2
+of u over fluid nodes; mass drift |M - M_ref| / M_ref with M the FP64
+on-device sum of the populations; drag = the z force of the momentum
+exchange (TLBM_Partition.cpp:143-208) summed over the obstacle's nodes.
+
+    python -m tlbm_amd.workloads porous  [--n 384] [--steps 5000]
+    python -m tlbm_amd.workloads spheroid [--scale 1] [--steps 10000] [--network FILE]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import geometry
+from .gpu import Lattice
+
+STORAGE_BYTES = {"f64": 8, "f32": 4, "f16": 2}
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_NETWORK = os.path.join(HERE, "closure_d3q19_256x128x128.json")
+
+
+def _log(msg: str) -> None:
+    print(f"[workload] {msg}", flush=True)
+
+
+def channel_types(obstacle: np.ndarray, walls: bool = True) -> np.ndarray:
+    """ndType (flat, global order) of a channel along z from an obstacle
+    mask shaped (nz, ny, nx): side walls x = 0, nx-1, y = 0, ny-1 and the
+    obstacle solid, inlet z = 0 and outlet z = nz-1 elsewhere
+    (FluidChannel.py:1358-1397, 1517-1538).  Built plane by plane."""
+    nz, ny, nx = obstacle.shape
+    nd = np.zeros((nz, ny, nx), np.uint8)
+    wall = np.zeros((ny, nx), bool)
+    if walls:
+        wall[:, 0] = wall[:, nx - 1] = True
+        wall[0, :] = wall[ny - 1, :] = True
+    for z in range(nz):
+        solid = obstacle[z] | wall
+        plane = nd[z]
+        plane[solid] = geometry.SOLID
+        if z == 0:
+            plane[~solid] = geometry.INLET
+        elif z == nz - 1:
+            plane[~solid] = geometry.OUTLET
+    return nd.ravel()
+
+
+def spheroid_mask(nx, ny, nz, xc, yc, zc, ab, c):
+    """Prolate spheroid on or inside (g <= 0, FluidChannel.py:378-393),
+    evaluated by broadcasting."""
+    x = (np.arange(nx, dtype=np.float64) - xc)[None, None, :]
+    y = (np.arange(ny, dtype=np.float64) - yc)[None, :, None]
+    z = (np.arange(nz, dtype=np.float64) - zc)[:, None, None]
+    out = np.empty((nz, ny, nx), bool)
+    for k in range(nz):
+        out[k] = (x[0] ** 2 + y[0] ** 2) / float(ab * ab) + z[k] ** 2 / float(c * c) - 1.0 <= 0.0
+    return out
+
+
+def timed_run(lat: Lattice, steps: int, chunk: int, tag: str) -> float:
+    """Runs `steps` steps in batches, returns the kernel milliseconds."""
+    ms, done = 0.0, 0
+    while done < steps:
+        k = min(chunk, steps - done)
+        ms += lat.step_timed(k)
+        done += k
+        _log(f"{tag}: {done}/{steps} steps, {ms / done:.3f} ms/step")
+    return ms
+
+
+def _rate(N, Q, storage, steps, ms):
+    mlups = N * steps / (ms * 1e3)
+    return {"mlups": mlups, "ms_per_step": ms / steps,
+            "hbm_gbs": mlups * 1e6 * 2 * Q * STORAGE_BYTES[storage] / 1e9,
+            "bytes_per_update": 2 * Q * STORAGE_BYTES[storage]}
+
+
+def _l2(a: dict, b: dict, mask) -> float:
+    ua = np.stack([a[k][mask] for k in ("ux", "uy", "uz")])
+    ub = np.stack([b[k][mask] for k in ("ux", "uy", "uz")])
+    return float(np.linalg.norm(ua - ub) / np.linalg.norm(ub))
+
+
+# ---------------------------------------------------------------------------
+
+
+def porous_case(n: int = 384, seed: int = 20251202, porosity: float = 0.65, z_margin: int = 4):
+    """Spheres added until the lattice (walls included) has the porosity.
+
+    The spheres stay z_margin nodes clear of the inlet and outlet planes:
+    an inlet node whose downstream neighbour is solid keeps receiving the
+    Zou-He inflow with nowhere to pass it on, and its density grows without
+    bound (to 6 in 60 steps on a 32^3 pack in the oracle, which follows the
+    reference's rules, D3Q27LatticeStructure.hpp:112-141)."""
+    solid = np.zeros((n, n, n), bool)
+    solid[:, :, 0] = solid[:, :, n - 1] = solid[:, 0, :] = solid[:, n - 1, :] = True
+    centres, radii = geometry.rsa_spheres(n, n, n, 6.0, 14.0, porosity, seed, solid_mask=solid,
+                                          z_margin=z_margin)
+    mask = geometry.sphere_pack_mask(n, n, n, centres, radii)
+    nd = channel_types(mask)
+    return nd, {"spheres": int(len(radii)), "porosity": float(np.mean(nd != geometry.SOLID))}
+
+
+def run_porous(n: int = 384, steps: int = 5000, chunk: int = 500, seed: int = 20251202,
+               ramp: int = 3000, checkpoints=(100, 500, 1000, 2000, 3000, 5000)):
+    """FP64 and FP32 lattices advanced side by side (both fit in HBM), the
+    velocity L2 and mass drift of FP32 against FP64 recorded at checkpoints;
+    each lattice's kernel time accumulated from its own HIP events."""
+    t0 = time.time()
+    nd, geo = porous_case(n, seed)
+    _log(f"porous {n}^3: {geo['spheres']} spheres, porosity {geo['porosity']:.4f}, "
+         f"{time.time() - t0:.1f} s")
+    N, Q = nd.size, 27
+    fluid = nd != geometry.SOLID
+    out = {"workload": f"porous_d3q27_{n}^3", "steps": steps, "seed": seed, "inlet_ramp": ramp,
+           **geo, "runs": {}, "checkpoints": []}
+    kw = dict(omega=1.0 / 0.505, u_lbm=0.05, cs=0.1)
+    lats = {st: Lattice(27, n, n, n, nd, storage=st, **kw) for st in ("f64", "f32")}
+    ms = {st: 0.0 for st in lats}
+    done, every = 0, 50
+    try:
+        while done < steps:
+            if done < ramp:
+                prof = np.full(n * n, 0.05 * min(1.0, (done + every) / ramp))
+                for lat in lats.values():
+                    lat.set_inlet_profile(prof)
+            k = min(every, steps - done)
+            for st, lat in lats.items():
+                ms[st] += lat.step_timed(k)
+            done += k
+            if done in checkpoints or done == steps:
+                m64, m32 = lats["f64"].macros(), lats["f32"].macros()
+                r64, r32 = lats["f64"].reduce()["mass"], lats["f32"].reduce()["mass"]
+                cp = {"step": done, "l2_velocity_vs_f64": _l2(m32, m64, fluid),
+                      "mass_drift_vs_f64": abs(r32 - r64) / r64,
+                      "max_abs_uz": float(np.nanmax(np.abs(m64["uz"][fluid]))),
+                      "rho_range": [float(np.nanmin(m64["rho"][fluid])),
+                                    float(np.nanmax(m64["rho"][fluid]))]}
+                out["checkpoints"].append(cp)
+                _log(f"porous step {done}: {cp}, ms/step f64 {ms['f64'] / done:.3f} "
+                     f"f32 {ms['f32'] / done:.3f}")
+                del m64, m32
+        for st, lat in lats.items():
+            r = _rate(N, Q, st, steps, ms[st])
+            r["mass"] = float(lat.reduce()["mass"])
+            out["runs"][st] = r
+    finally:
+        for lat in lats.values():
+            lat.close()
+    last = out["checkpoints"][-1]
+    out["runs"]["f32"]["l2_velocity_vs_f64"] = last["l2_velocity_vs_f64"]
+    out["runs"]["f32"]["mass_drift_vs_f64"] = last["mass_drift_vs_f64"]
+    _log(f"porous runs: {out['runs']}")
+    return out
+
+
+# ---------------------------------------------------------------------------
+
+
+def spheroid_case(scale: int = 1):
+    """(nd, spheroid mask flat) of the spheroid channel at 1/scale size."""
+    nx, ny, nz = 1024 // scale, 512 // scale, 512 // scale
+    mask = spheroid_mask(nx, ny, nz, nx / 2, ny / 2, nz / 2, 32 / scale, 96 / scale)
+    return (nx, ny, nz), channel_types(mask), mask.ravel()
+
+
+def _spheroid_run(dims, nd, body, storage, tau, steps, chunk, closure=None, cs=0.1):
+    nx, ny, nz = dims
+    with Lattice(19, nx, ny, nz, nd, storage=storage, omega=1.0 / tau, u_lbm=0.05,
+                 cs=0.0 if closure is not None else cs) as lat:
+        if closure is not None:
+            lat.set_closure(*closure)
+        tag = f"spheroid {nx}x{ny}x{nz} {storage}{'+network' if closure else ''}"
+        ms = timed_run(lat, steps, chunk, tag)
+        m = lat.macros()
+        red = lat.reduce()
+        fz = float(np.sum(lat.forces()["Fz"][body]))
+    r = _rate(nd.size, 19, storage, steps, ms)
+    r.update({"drag": fz, "mass": red["mass"]})
+    _log(f"{tag}: {r}")
+    return r, m
+
+
+def run_spheroid(scale: int = 1, steps: int = 10000, chunk: int = 1000,
+                 network: str = DEFAULT_NETWORK, compare_down: bool = True):
+    from .closure import load_network
+    closure = load_network(network)
+    out = {"workload": "spheroid_d3q19", "network": os.path.basename(network), "full": {},
+           "downsampled": {}}
+    dims, nd, body = spheroid_case(scale)
+    fluid = nd != geometry.SOLID
+    out["full"]["lattice"] = list(dims)
+    out["full"]["steps"] = steps
+    tau = 0.51
+    r16, m16 = _spheroid_run(dims, nd, body, "f16", tau, steps, chunk, closure)
+    r32, m32 = _spheroid_run(dims, nd, body, "f32", tau, steps, chunk)
+    r16["l2_velocity_vs_f32_smagorinsky"] = _l2(m16, m32, fluid)
+    r16["drag_rel_diff_vs_f32_smagorinsky"] = abs(r16["drag"] - r32["drag"]) / abs(r32["drag"])
+    out["full"]["f16_network"] = r16
+    out["full"]["f32_smagorinsky"] = r32
+    del m16, m32
+    if compare_down:
+        dims2, nd2, body2 = spheroid_case(2 * scale)
+        fluid2 = nd2 != geometry.SOLID
+        tau2, steps2 = 0.5 + (tau - 0.5) / 2.0, steps // 2
+        out["downsampled"]["lattice"] = list(dims2)
+        out["downsampled"]["steps"] = steps2
+        r64, m64 = _spheroid_run(dims2, nd2, body2, "f64", tau2, steps2, chunk)
+        out["downsampled"]["f64_smagorinsky"] = r64
+        for storage, cl, key in (("f32", None, "f32_smagorinsky"), ("f16", closure, "f16_network"),
+                                 ("f16", None, "f16_smagorinsky")):
+            r, m = _spheroid_run(dims2, nd2, body2, storage, tau2, steps2, chunk, cl)
+            r["l2_velocity_vs_f64"] = _l2(m, m64, fluid2)
+            r["mass_drift_vs_f64"] = abs(r["mass"] - r64["mass"]) / r64["mass"]
+            r["drag_rel_diff_vs_f64"] = abs(r["drag"] - r64["drag"]) / abs(r64["drag"])
+            out["downsampled"][key] = r
+    return out
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description="measured workloads")
+    sub = ap.add_subparsers(dest="which", required=True)
+    p = sub.add_parser("porous")
+    p.add_argument("--n", type=int, default=384)
+    p.add_argument("--steps", type=int, default=5000)
+    p.add_argument("--seed", type=int, default=20251202)
+    p.add_argument("--ramp", type=int, default=3000,
+                   help="steps over which the inlet velocity rises to 0.05 (0: from the start)")
+    s = sub.add_parser("spheroid")
+    s.add_argument("--scale", type=int, default=1, help="lattice 1024/scale x 512/scale^2")
+    s.add_argument("--steps", type=int, default=10000)
+    s.add_argument("--network", default=DEFAULT_NETWORK)
+    s.add_argument("--no-downsampled", action="store_true")
+    for q in (p, s):
+        q.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    if a.which == "porous":
+        rep = run_porous(a.n, a.steps, seed=a.seed, ramp=a.ramp)
+    else:
+        rep = run_spheroid(a.scale, a.steps, network=a.network,
+                           compare_down=not a.no_downsampled)
+    text = json.dumps(rep, indent=1)
+    if a.out:
+        with open(a.out, "w") as fh:
+            fh.write(text)
+    print(text)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
