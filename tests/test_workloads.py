@@ -1,0 +1,60 @@
+"""The workload runners' geometry (tlbm_amd/workloads.py) on the CPU, and
+short GPU runs of each runner."""
+import numpy as np
+import pytest
+
+from tlbm_amd import geometry, workloads
+
+
+def test_channel_types_match_generic_builder():
+    nx, ny, nz = 12, 10, 14
+    ob = geometry.sphere(5.5, 4.5, 7.0, 3.0)
+    x, y, z = geometry.grid(nx, ny, nz)
+    mask = ob(x, y, z)
+    want = geometry.channel(nx, ny, nz, obstacles=[ob])
+    assert np.array_equal(workloads.channel_types(mask), want)
+    want_open = geometry.channel(nx, ny, nz, walls=(), obstacles=[ob])
+    assert np.array_equal(workloads.channel_types(mask, walls=False), want_open)
+
+
+def test_spheroid_mask_matches_generic_rule():
+    nx, ny, nz = 32, 16, 16
+    got = workloads.spheroid_mask(nx, ny, nz, nx / 2, ny / 2, nz / 2, 4.0, 6.0)
+    x, y, z = geometry.grid(nx, ny, nz)
+    want = geometry.prolate_spheroid(nx / 2, ny / 2, nz / 2, 4.0, 6.0)(x, y, z)
+    assert np.array_equal(got, want) and got.any()
+
+
+def test_porous_case_reaches_porosity_and_clears_the_ends():
+    nd, geo = workloads.porous_case(48, seed=3)
+    assert abs(geo["porosity"] - 0.65) < 0.01
+    g = nd.reshape(48, 48, 48)
+    # no sphere within 4 nodes of the inlet and outlet: those planes are
+    # fluid apart from the side walls
+    for zplane in (0, 1, 2, 3, 4, 43, 44, 45, 46, 47):
+        inner = g[zplane, 1:-1, 1:-1]
+        assert not np.any(inner == geometry.SOLID), zplane
+    assert np.all(g[0, 1:-1, 1:-1] == geometry.INLET)
+    assert np.all(g[-1, 1:-1, 1:-1] == geometry.OUTLET)
+
+
+@pytest.mark.gpu
+def test_porous_runner_small(gpu_lib):
+    rep = workloads.run_porous(n=48, steps=300, chunk=100, seed=3, ramp=200,
+                               checkpoints=(100, 300))
+    for st in ("f64", "f32"):
+        assert rep["runs"][st]["mlups"] > 0
+    first = rep["checkpoints"][0]
+    assert np.isfinite(first["l2_velocity_vs_f64"]) and first["l2_velocity_vs_f64"] < 1e-3
+    assert first["mass_drift_vs_f64"] < 1e-5
+
+
+@pytest.mark.gpu
+def test_spheroid_runner_small(gpu_lib):
+    rep = workloads.run_spheroid(scale=16, steps=200, chunk=100)
+    full = rep["full"]
+    assert full["lattice"] == [64, 32, 32]
+    assert np.isfinite(full["f16_network"]["l2_velocity_vs_f32_smagorinsky"])
+    down = rep["downsampled"]
+    assert down["f32_smagorinsky"]["l2_velocity_vs_f64"] < 1e-4
+    assert down["f16_smagorinsky"]["l2_velocity_vs_f64"] < 1e-2

@@ -578,7 +578,16 @@ def main(argv=None) -> int:
     ap.add_argument("--dump-snapshots", type=int, default=8)
     ap.add_argument("--save", default=None,
                     help="write the trained network (params, feature and output scale) as JSON")
+    ap.add_argument("--cost-only", default=None, metavar="NETWORK",
+                    help="only time the saved network's collide against Smagorinsky "
+                         "(D3Q19 128x128x512, FP32 and FP16)")
     a = ap.parse_args(argv)
+    if a.cost_only:
+        net = load_network(a.cost_only)
+        cost = {f"128x128x512_{st}": step_cost((128, 128, 512), net, storage=st)
+                for st in ("f32", "f16")}
+        print(json.dumps(cost, indent=1))
+        return 0
     case = scaled_case(a.scale, snapshots=a.snapshots) if a.scale != 1.0 else \
         ClosureCase(snapshots=a.snapshots)
     shapes = [case.coarse, (128, 128, 512)] if a.cost else []

@@ -47,6 +47,7 @@ constexpr int kBlock = 256;  // 4 wave64 per workgroup
 constexpr int kRed = 6;      // values per FP64 reduction partial
 constexpr int kFilterPlanes = 16;  // box filter output planes
 constexpr int kClosureBlock = 512;  // workgroup of the closure collide
+constexpr bool kClosureNTLoad = true;  // non-temporal population loads there
 
 // Launch arguments, passed by value (kernarg segment, scalar registers).
 struct StepArgs {
@@ -145,8 +146,9 @@ template <> struct StorageT<ST_F16> {
 // 128x128x512 lattice: one node per lane everywhere, FP64 in 1024-thread
 // workgroups with non-temporal loads (ahead of a two-nodes-per-lane 16-byte
 // variant: 20.9 against 19.8 GLUPS, and 13.5 against 12.0 for D3Q27),
-// FP32 and FP16 in 512-thread workgroups, FP16 with non-temporal loads.
-// Every kernel stores non-temporally.
+// FP32 and FP16 in 512-thread workgroups with plain loads (FP16: 109.7
+// against 111.5 us per step with non-temporal loads).  Every kernel stores
+// non-temporally.
 template <int ST> struct Tune;
 template <> struct Tune<ST_F64> {
     static constexpr int kBS = 1024;
@@ -158,7 +160,7 @@ template <> struct Tune<ST_F32> {
 };
 template <> struct Tune<ST_F16> {
     static constexpr int kBS = 512;
-    static constexpr bool kNTLoad = true;
+    static constexpr bool kNTLoad = false;
 };
 
 // Workgroup -> node block.  The dispatcher deals workgroups round-robin to
