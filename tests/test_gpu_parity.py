@@ -88,6 +88,37 @@ def test_fp64_random_state_matches_oracle(gpu_lib, geom, q, cs):
                 assert rel_err(fg[key], fo[key]) <= TOL64, key
 
 
+@pytest.mark.parametrize("q", [15, 19, 27])
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
+def test_relative_addressing_lattice_matches_oracle(gpu_lib, q, storage):
+    """An odd-sized lattice (33x17x40, walls, spheres, inlet/outlet) where
+    most workgroups take the 32-bit relative gather (planes 1..nz-2) and the
+    rest the general one, from a random state: FP64 to the FP64 bar, FP32
+    and FP16 (deviation storage) within their precision."""
+    nx, ny, nz = 33, 17, 40
+    nd = geometry.channel(nx, ny, nz, obstacles=[geometry.sphere(16, 8, 12, 5),
+                                                 geometry.sphere(5, 9, 30, 3.5)])
+    kw = dict(omega=1.6, u_lbm=0.04, rho_lbm=1.0, cs=0.125)
+    print(f"relative addressing q={q} {storage}", flush=True)
+    o = Oracle(q, nx, ny, nz, nd, **kw)
+    f0 = _random_state(np.random.default_rng(q + 7), o.weights(), nx * ny * nz, amp=0.05)
+    o.set_populations(f0)
+    tol = {"f64": TOL64, "f32": 1e-5, "f16": 2e-3}[storage]
+    with Lattice(q, nx, ny, nz, nd, storage=storage, **kw) as lat:
+        lat.set_populations(f0)
+        for nsteps in (1, 5):
+            o.step(nsteps)
+            lat.step(nsteps)
+            mo, mg = o.macros(), lat.macros()
+            fluid = nd != 1
+            for key in ("ux", "uy", "uz"):
+                err = np.linalg.norm(mg[key][fluid] - mo[key][fluid]) / np.linalg.norm(mo[key][fluid])
+                assert err <= tol, (nsteps, key, err)
+            assert rel_err(mg["rho"], mo["rho"]) <= tol
+            if storage == "f64":
+                assert rel_err(lat.populations(), o.populations()) <= TOL64
+
+
 def test_fp64_dynamics_other_than_bgk_does_not_relax(gpu_lib):
     nx, ny, nz, nd = GEOMS["channel"]()
     kw = dict(omega=1.6, u_lbm=0.04, rho_lbm=1.0, cs=0.1, dynamics=2)
