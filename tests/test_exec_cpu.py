@@ -121,3 +121,35 @@ def test_restart_tool_reorders_and_writes_float32(tmp_path, dtype, flag):
     # the reference's script opens with 'w-': an existing file is not overwritten
     r2 = subprocess.run([RESTART, "3", flag], cwd=d, capture_output=True, text=True)
     assert r2.returncode == 1
+
+
+def test_ndtype_sidecar_is_read_and_validated(tmp_path):
+    p = good_params()
+    nd = nd_for(p).astype(int)
+    d = case_dir(tmp_path, p, nd)
+    bad = nd.copy()
+    bad[7] = 4
+    io_formats.write_ndtype_bin(os.path.join(d, "ndType.bin"), bad)  # newer than ndType.lbm
+    r = run_exec(d)
+    assert r.returncode == 1 and "ndType.bin: node 7 has type 4" in r.stderr
+    io_formats.write_ndtype_bin(os.path.join(d, "ndType.bin"), nd[:-1])
+    r = run_exec(d)
+    assert r.returncode == 1 and "ndType.bin has" in r.stderr
+    with open(os.path.join(d, "ndType.bin"), "wb") as fh:
+        fh.write(b"NOTASIDE" + bytes(8))
+    r = run_exec(d)
+    assert r.returncode == 1 and "not an ndType sidecar" in r.stderr
+
+
+def test_stale_ndtype_sidecar_is_ignored(tmp_path):
+    p = good_params()
+    nd = nd_for(p).astype(int)
+    d = case_dir(tmp_path, p, nd)
+    io_formats.write_ndtype_bin(os.path.join(d, "ndType.bin"), nd[:-1])  # would be rejected
+    bad = nd.copy()
+    bad[7] = 4
+    io_formats.write_int_lines(os.path.join(d, "ndType.lbm"), bad)
+    t = os.stat(os.path.join(d, "ndType.bin")).st_mtime
+    os.utime(os.path.join(d, "ndType.lbm"), (t + 10, t + 10))  # the text file is newer
+    r = run_exec(d)
+    assert r.returncode == 1 and "ndType.lbm: node 7 has type 4" in r.stderr

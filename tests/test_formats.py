@@ -1,5 +1,7 @@
 """Host-side inputs: params.lbm / ndType.lbm / parts.lbm and the node
 classification rules, checked by brute force on small lattices."""
+import os
+
 import numpy as np
 
 from tlbm_amd import geometry, io_formats
@@ -97,3 +99,24 @@ def test_rsa_pack_reaches_lattice_porosity():
     # stops with the first sphere that crosses the target
     last = geometry.sphere_pack_mask(n, n, n, c[-1:], r[-1:]).sum()
     assert 0.8 - last / n ** 3 <= porosity <= 0.8
+
+
+def test_int_lines_match_printf_and_read_back(tmp_path):
+    rng = np.random.default_rng(4)
+    for v in (rng.integers(0, 6, 500), rng.integers(-5000, 200000, 2000), np.array([0]),
+              np.array([-1, 10, 9, 100000000])):
+        want = "".join("%d \n" % x for x in v.tolist()).encode()
+        assert io_formats.int_lines_bytes(v) == want
+        path = os.path.join(str(tmp_path), "x.lbm")
+        io_formats.write_int_lines(path, v)
+        assert np.array_equal(io_formats.read_int_lines(path), v)
+
+
+def test_ndtype_sidecar_round_trip(tmp_path):
+    nd = geometry.channel(7, 5, 9, obstacles=[geometry.sphere(3, 2, 4, 2)])
+    path = os.path.join(str(tmp_path), "ndType.bin")
+    io_formats.write_ndtype_bin(path, nd)
+    assert np.array_equal(io_formats.read_ndtype_bin(path), nd)
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"TLBMNDT1" and int.from_bytes(raw[8:16], "little") == nd.size
+    assert len(raw) == 16 + nd.size

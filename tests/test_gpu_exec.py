@@ -156,3 +156,24 @@ def test_exec_restart_from_dump(gpu_lib, tmp_path):
     got = read_dump(d, 0, np.float64)
     for f, k in (("ux", "ux"), ("uy", "uy"), ("uz", "uz"), ("density", "rho")):
         assert rel_err(got[f], ref[k]) <= TOL64, f
+
+
+def test_exec_reads_the_binary_node_type_sidecar(gpu_lib, tmp_path):
+    """The configs' 24x24x96 D3Q15 case written with ndType.bin (tlbm_amd.cases)
+    and the text ndType.lbm replaced by garbage the driver would reject: the
+    run must use the sidecar and match the reference's 4-rank dumps."""
+    from tlbm_amd import cases
+    case = [c for c in goldens.load_cases() if c.name.startswith("channel_d3q15_24x24x96_f64")][0]
+    d = str(tmp_path)
+    p, nd = cases.write("channel", d, parts=4, binary=True, steps=case.params["numTs"],
+                        plot_freq=case.params["numTs"])
+    bin_path = os.path.join(d, "ndType.bin")
+    t = os.stat(bin_path).st_mtime
+    with open(os.path.join(d, "ndType.lbm"), "w") as fh:
+        fh.write("7\n")                       # would fail if it were read
+    os.utime(os.path.join(d, "ndType.lbm"), (t - 10, t - 10))
+    r = run_exec(d, storage="f64")
+    assert r.returncode == 0, r.stderr
+    got = read_dump(d, 0, np.float64)
+    for f, ref in case.out.items():
+        assert rel_err(got[f], ref) <= TOL64, f

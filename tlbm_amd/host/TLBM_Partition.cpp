@@ -1,6 +1,8 @@
 // TLBM_Partition.cpp -- the partition object over the GPU C-ABI.
 #include "TLBM_Partition.h"
 
+#include <sys/stat.h>
+
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
@@ -134,12 +136,60 @@ TLBM_Partition::TLBM_Partition(int rank, int size) : rank_(rank) {
 
 TLBM_Partition::~TLBM_Partition() { tlbm_gpu_destroy(gpu_); }
 
+namespace {
+
+bool valid_type(long v) { return v == 0 || v == 1 || v == 2 || v == 3 || v == 5; }
+
+// ndType.bin: "TLBMNDT1", the node count (little-endian uint64), one byte
+// per node in global order.  Used in place of ndType.lbm when it exists and
+// is not older than the text file (a text file written after it wins).
+bool read_ndtype_sidecar(std::vector<uint8_t> &nd, size_t N) {
+    struct stat sb, st;
+    if (stat("ndType.bin", &sb) != 0) return false;
+    if (stat("ndType.lbm", &st) == 0) {
+        const bool older = sb.st_mtim.tv_sec < st.st_mtim.tv_sec ||
+                           (sb.st_mtim.tv_sec == st.st_mtim.tv_sec &&
+                            sb.st_mtim.tv_nsec < st.st_mtim.tv_nsec);
+        if (older) return false;
+    }
+    FILE *fp = std::fopen("ndType.bin", "rb");
+    if (!fp) throw std::runtime_error("Could not open ndType.bin");
+    char magic[8];
+    unsigned char cnt[8];
+    if (std::fread(magic, 1, 8, fp) != 8 || std::memcmp(magic, "TLBMNDT1", 8) != 0 ||
+        std::fread(cnt, 1, 8, fp) != 8) {
+        std::fclose(fp);
+        throw std::runtime_error("ndType.bin: not an ndType sidecar");
+    }
+    uint64_t n = 0;
+    for (int k = 7; k >= 0; --k) n = (n << 8) | cnt[k];
+    if (n != N) {
+        std::fclose(fp);
+        throw std::runtime_error("ndType.bin has " + std::to_string(n) + " entries, expected " +
+                                 std::to_string(N));
+    }
+    nd.assign(N, 0);
+    const size_t got = std::fread(nd.data(), 1, N, fp);
+    const bool extra = std::fgetc(fp) != EOF;
+    std::fclose(fp);
+    if (got != N || extra)
+        throw std::runtime_error("ndType.bin: payload is not " + std::to_string(N) + " bytes");
+    for (size_t i = 0; i < N; ++i)
+        if (!valid_type(nd[i]))
+            throw std::invalid_argument("ndType.bin: node " + std::to_string(i) + " has type " +
+                                        std::to_string(int(nd[i])) + "; valid types are 0,1,2,3,5");
+    return true;
+}
+
+}  // namespace
+
 void TLBM_Partition::load_ndType(std::vector<uint8_t> &nd) {
     const size_t N = size_t(thisProblem.nx) * thisProblem.ny * thisProblem.nz;
+    if (read_ndtype_sidecar(nd, N)) return;
     nd.assign(N, 0);
     const size_t n = for_each_int("ndType.lbm", [&](size_t i, long v) {
         if (i >= N) throw std::runtime_error("ndType.lbm has more than nx*ny*nz entries");
-        if (!(v == 0 || v == 1 || v == 2 || v == 3 || v == 5))
+        if (!valid_type(v))
             throw std::invalid_argument("ndType.lbm: node " + std::to_string(i) + " has type " +
                                         std::to_string(v) + "; valid types are 0,1,2,3,5");
         nd[i] = uint8_t(v);

@@ -91,10 +91,73 @@ def read_params(path: str) -> Params:
     return Params(**kw)
 
 
+def int_lines_bytes(values) -> bytes:
+    """The text "%d \n" per value (non-negative or negative ints), built
+    with array arithmetic: 268M node types take seconds, not minutes."""
+    raw = np.asarray(values).ravel()
+    if raw.size == 0:
+        return b""
+    if raw.min() >= 0 and raw.max() <= 9:        # node types: 3 bytes a line
+        out = np.empty((raw.size, 3), np.uint8)
+        out[:, 0] = raw.astype(np.uint8) + ord("0")
+        out[:, 1] = ord(" ")
+        out[:, 2] = ord("\n")
+        return out.tobytes()
+    v = raw.astype(np.int64)
+    neg = v < 0
+    a = np.abs(v)
+    ndig = np.ones(v.size, np.int64)
+    t = a // 10
+    while np.any(t > 0):
+        ndig += t > 0
+        t //= 10
+    width = ndig + neg + 2                      # digits, sign, " \n"
+    end = np.cumsum(width)
+    start = end - width
+    out = np.empty(int(end[-1]), np.uint8)
+    out[end - 1] = ord("\n")
+    out[end - 2] = ord(" ")
+    out[start[neg]] = ord("-")
+    first = start + neg                         # position of the leading digit
+    rem = a.copy()
+    for k in range(int(ndig.max()) - 1, -1, -1):
+        has = ndig > k                          # values with a digit at place 10^k
+        pos = first[has] + (ndig[has] - 1 - k)
+        out[pos] = ord("0") + (rem[has] // 10 ** k) % 10
+    return out.tobytes()
+
+
 def write_int_lines(path: str, values) -> None:
-    arr = np.asarray(values, dtype=np.int64).ravel()
-    with open(path, "w") as fh:
-        fh.write("".join("%d \n" % v for v in arr.tolist()))
+    with open(path, "wb") as fh:
+        fh.write(int_lines_bytes(values))
+
+
+# Binary sidecar of ndType.lbm: magic, node count (little-endian uint64),
+# then one byte per node in global order.  The drop-in driver reads it in
+# place of the text file when it is present and not older than ndType.lbm.
+NDTYPE_BIN_MAGIC = b"TLBMNDT1"
+
+
+def write_ndtype_bin(path: str, nd_type) -> None:
+    nd = np.asarray(nd_type).ravel()
+    if nd.size and (nd.min() < 0 or nd.max() > 255):
+        raise ValueError("node types must fit in a byte")
+    with open(path, "wb") as fh:
+        fh.write(NDTYPE_BIN_MAGIC)
+        fh.write(np.uint64(nd.size).tobytes())
+        fh.write(nd.astype(np.uint8).tobytes())
+
+
+def read_ndtype_bin(path: str) -> np.ndarray:
+    with open(path, "rb") as fh:
+        head = fh.read(16)
+        if len(head) != 16 or head[:8] != NDTYPE_BIN_MAGIC:
+            raise ValueError(f"{path}: not an ndType sidecar")
+        n = int(np.frombuffer(head[8:], np.uint64)[0])
+        nd = np.frombuffer(fh.read(), np.uint8)
+    if nd.size != n:
+        raise ValueError(f"{path}: {nd.size} node types, header says {n}")
+    return nd.copy()
 
 
 def read_int_lines(path: str) -> np.ndarray:
@@ -107,14 +170,17 @@ def slab_parts(nx: int, ny: int, nz: int, nparts: int) -> np.ndarray:
     return (z * nparts // nz).astype(np.int64)
 
 
-def write_case(directory: str, p: Params, nd_type, parts=None) -> None:
-    """params.lbm, ndType.lbm and parts.lbm for one run directory."""
+def write_case(directory: str, p: Params, nd_type, parts=None, binary: bool = False) -> None:
+    """params.lbm, ndType.lbm and parts.lbm for one run directory (and,
+    with binary, the ndType.bin sidecar, written after the text file)."""
     os.makedirs(directory, exist_ok=True)
     write_params(os.path.join(directory, "params.lbm"), p)
     write_int_lines(os.path.join(directory, "ndType.lbm"), nd_type)
     if parts is None:
         parts = np.zeros(p.num_nodes, dtype=np.int64)
     write_int_lines(os.path.join(directory, "parts.lbm"), parts)
+    if binary:
+        write_ndtype_bin(os.path.join(directory, "ndType.bin"), nd_type)
 
 
 def read_bdat(directory: str, field: str, k: int, dtype) -> np.ndarray:
