@@ -125,7 +125,10 @@ template <> struct StorageT<ST_F16> {
     static constexpr int kCB = 14;  // 32 KiB chunks
     using T = float;
     static constexpr bool kShift = true;
-    __device__ static inline T in(st r, int s, const StepArgs &a) { return float(r) * a.h_unscale[s]; }
+    // one mixed-precision FMA (f16 operand, f32 scale)
+    __device__ static inline T in(st r, int s, const StepArgs &a) {
+        return __builtin_fmaf(float(r), a.h_unscale[s], 0.0f);
+    }
     __device__ static inline st out(T v, int s, const StepArgs &a) { return _Float16(v * a.h_scale[s]); }
     __device__ static inline T load(const st *p, int s, const StepArgs &a) {
         return float(*p) * a.h_unscale[s];
@@ -339,6 +342,8 @@ __device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho,
         if constexpr (SHIFT) {
             dlt = r + T(a.dw);
             rho = T(1) + dlt;
+            // a correctly rounded 1/rho: a 1-ulp reciprocal biases u and
+            // doubles the FP32 velocity error against FP64 over 1000 steps
             const T inv = T(1) / rho;
             ux = mx * inv;
             uy = my * inv;
@@ -420,16 +425,26 @@ __device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho,
         }
     }
     // compute_equilibrium, LatticeStructure.hpp:92-103
-    // (SHIFT: f_eq - w = w (delta + rho (cu + cu^2/2 - 3u^2/2)))
     {
         const T usq = ux * ux + uy * uy + uz * uz;
+        if constexpr (SHIFT) {
+            // f_eq - w = w (delta + rho (cu + cu^2/2 - 3u^2/2)), cu = 3 e.u,
+            // rewritten with t = e.(3 rho u) = rho cu:  w (A + t + t^2 / (2 rho)),
+            // A = delta - 3/2 rho u^2 (the same polynomial, fewer operations)
+            const T A = dlt - T(1.5) * rho * usq;
+            const T h = T(0.5) * __builtin_amdgcn_rcpf(rho);
+            const T Ux = T(3) * rho * ux, Uy = T(3) * rho * uy, Uz = T(3) * rho * uz;
 #pragma unroll
-        for (int s = 0; s < Q; ++s) {
-            const T cu = (s == 0) ? T(0) : T(3) * edot<T>(L::ex[s], L::ey[s], L::ez[s], ux, uy, uz);
-            if constexpr (SHIFT)
-                feq[s] = w[s] * (dlt + rho * (cu + T(0.5) * (cu * cu) - T(1.5) * usq));
-            else
+            for (int s = 0; s < Q; ++s) {
+                const T t = (s == 0) ? T(0) : edot<T>(L::ex[s], L::ey[s], L::ez[s], Ux, Uy, Uz);
+                feq[s] = w[s] * (A + t * (T(1) + t * h));
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < Q; ++s) {
+                const T cu = (s == 0) ? T(0) : T(3) * edot<T>(L::ex[s], L::ey[s], L::ez[s], ux, uy, uz);
                 feq[s] = w[s] * rho * (T(1) + cu + T(0.5) * (cu * cu) - T(1.5) * usq);
+            }
         }
     }
     if (t == 2) {
