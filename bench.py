@@ -88,12 +88,6 @@ def traffic_bytes(storage):
         return None
 
 
-def _write_lines(path, arr):
-    with open(path, "w") as fh:
-        fh.write(" \n".join(map(str, np.asarray(arr).tolist())))
-        fh.write(" \n")
-
-
 def cpu_reference(nd, steps, cores, timeout):
     """The reference executable (real=float) on `cores` MPI ranks with 1-D
     z-slab partitions; returns its own `Estimated LPU/s` in MLUPS."""
@@ -106,8 +100,8 @@ def cpu_reference(nd, steps, cores, timeout):
                           omega=1.0 / TAU, nx=NX, ny=NY, nz=NZ)
     with tempfile.TemporaryDirectory(prefix="tlbm_cpu_") as d:
         io_formats.write_params(os.path.join(d, "params.lbm"), p)
-        _write_lines(os.path.join(d, "ndType.lbm"), nd)
-        _write_lines(os.path.join(d, "parts.lbm"), io_formats.slab_parts(NX, NY, NZ, cores))
+        io_formats.write_int_lines(os.path.join(d, "ndType.lbm"), nd)
+        io_formats.write_int_lines(os.path.join(d, "parts.lbm"), io_formats.slab_parts(NX, NY, NZ, cores))
         r = subprocess.run([mpiexec, "-launcher", "fork", "-n", str(cores), exe], cwd=d,
                            capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:

@@ -30,7 +30,7 @@ struct Launch {
                     return hipErrorInvalidValue;
                 } else {
                     constexpr int CBS = kClosureBlock;
-                    hipLaunchKernelGGL((k_lbm_closure<L, ST, CBS, kClosureNTLoad>),
+                    hipLaunchKernelGGL((k_lbm_closure<L, ST, CBS, kClosureNTLoad<ST>>),
                                        dim3((a.N + CBS - 1) / CBS), dim3(CBS), 0, s, in,
                                        out, nd, chunk_type, uyh, a);
                 }

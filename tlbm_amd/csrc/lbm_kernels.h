@@ -47,7 +47,9 @@ constexpr int kBlock = 256;  // 4 wave64 per workgroup
 constexpr int kRed = 6;      // values per FP64 reduction partial
 constexpr int kFilterPlanes = 16;  // box filter output planes
 constexpr int kClosureBlock = 512;  // workgroup of the closure collide
-constexpr bool kClosureNTLoad = true;  // non-temporal population loads there
+// non-temporal population loads in the closure collide: FP16 only (FP32:
+// 221.8 against 217.6 us per step with them)
+template <int ST> constexpr bool kClosureNTLoad = (ST == ST_F16);
 
 // Launch arguments, passed by value (kernarg segment, scalar registers).
 struct StepArgs {
@@ -148,17 +150,18 @@ template <> struct StorageT<ST_F16> {
 // Launch choices per storage, measured on MI355X with the D3Q19
 // 128x128x512 lattice: one node per lane everywhere, FP64 in 1024-thread
 // workgroups with non-temporal loads (ahead of a two-nodes-per-lane 16-byte
-// variant: 20.9 against 19.8 GLUPS, and 13.5 against 12.0 for D3Q27),
-// FP32 and FP16 in 512-thread workgroups with plain loads (FP16: 109.7
-// against 111.5 us per step with non-temporal loads).  Every kernel stores
-// non-temporally.
+// variant: 20.9 against 19.8 GLUPS, and 13.5 against 12.0 for D3Q27; 1024
+// threads ahead of 512), FP32 in 256-thread workgroups (203 against 205 us
+// with 512), FP16 in 512-thread workgroups, both with plain loads (FP16:
+// 109.7 against 111.5 us per step with non-temporal loads).  Every kernel
+// stores non-temporally.
 template <int ST> struct Tune;
 template <> struct Tune<ST_F64> {
     static constexpr int kBS = 1024;
     static constexpr bool kNTLoad = true;
 };
 template <> struct Tune<ST_F32> {
-    static constexpr int kBS = 512;
+    static constexpr int kBS = 256;
     static constexpr bool kNTLoad = false;
 };
 template <> struct Tune<ST_F16> {
