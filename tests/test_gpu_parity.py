@@ -119,6 +119,32 @@ def test_relative_addressing_lattice_matches_oracle(gpu_lib, q, storage):
                 assert rel_err(lat.populations(), o.populations()) <= TOL64
 
 
+@pytest.mark.parametrize("dims", [(1, 1, 8), (1, 5, 7), (3, 1, 9), (2, 2, 2), (4, 3, 1),
+                                  (5, 4, 2), (64, 1, 40)])
+@pytest.mark.parametrize("q", [15, 19, 27])
+def test_degenerate_shapes_match_oracle(gpu_lib, dims, q):
+    """Lattices with a dimension of 1 or 2 (the periodic wrap finds the node
+    itself or its only neighbour; division by 1; fewer than three planes, so
+    no workgroup takes the relative gather), from a random state, FP64 and
+    FP32."""
+    nx, ny, nz = dims
+    nd = geometry.periodic_box(nx, ny, nz)
+    kw = dict(omega=1.6, u_lbm=0.0, rho_lbm=1.0, cs=0.125)
+    print(f"degenerate {dims} q={q}", flush=True)
+    o = Oracle(q, nx, ny, nz, nd, **kw)
+    f0 = _random_state(np.random.default_rng(sum(dims) + q), o.weights(), nx * ny * nz)
+    o.set_populations(f0)
+    with Lattice(q, nx, ny, nz, nd, storage="f64", **kw) as a, \
+            Lattice(q, nx, ny, nz, nd, storage="f32", **kw) as b:
+        a.set_populations(f0)
+        b.set_populations(f0)
+        o.step(3)
+        a.step(3)
+        b.step(3)
+        assert rel_err(a.populations(), o.populations()) <= TOL64
+        assert rel_err(b.populations(), o.populations()) <= 1e-5
+
+
 def test_fp64_dynamics_other_than_bgk_does_not_relax(gpu_lib):
     nx, ny, nz, nd = GEOMS["channel"]()
     kw = dict(omega=1.6, u_lbm=0.04, rho_lbm=1.0, cs=0.1, dynamics=2)
