@@ -31,10 +31,22 @@ learned replacement and judges it against that form:
    Smagorinsky fixed point, then, in refit rounds, at the network's.
 5. Training: the HIP trainer (Adam, mean squared error of softplus(z)
    against dtau*/out_scale).
-6. Evaluation: coarse runs with the network and with Smagorinsky (cs of the
-   fine run) against the filtered fine fields -- short horizons started
-   from filtered snapshots, and time statistics (mean and rms velocity)
-   of runs started from rest -- plus the per-step cost of both.
+6. Calibration, a posteriori and on the training window only: a grid over
+   the output multiplier m (out_scale -> m out_scale) and a base offset c
+   of the relaxation time (the coarse lattice runs at tau0 + c, the
+   network adds its increment on top).  The choice minimises the sum of
+   the two training-window errors relative to Smagorinsky's: short
+   horizons started from training snapshots, and the mean velocity of a
+   long run over the training window.
+7. Evaluation on the held-out window (the second half of the snapshots):
+   coarse runs with the network and with Smagorinsky (cs of the fine run)
+   against the filtered fine fields -- short horizons started from
+   held-out snapshots, and time statistics (mean and rms velocity) of one
+   long run from uniform flow, per window -- plus the per-step cost of
+   both.
+
+A closure is None (Smagorinsky) or (params, feature_scale, output_scale[,
+tau_offset]).
 
 Everything runs through the C-ABI (tlbm_amd.gpu); numpy only forms the
 coarse-grid targets and features.
@@ -68,9 +80,9 @@ class ClosureCase:
     perturb: float = 0.01
     seed: int = 20251202
     spinup: int = 6400      # fine steps before the first snapshot
-    snapshots: int = 50
+    snapshots: int = 100
     every: int = 100        # fine steps between snapshots (even)
-    holdout: int = 12       # last snapshots kept out of training
+    holdout: int = 50       # last snapshots kept out of training and calibration
     margin: int = 2         # coarse cells kept clear of solids and z ends
 
     @property
@@ -360,8 +372,16 @@ def train_closure(case: ClosureCase, data: dict, *, steps=(2000, 1500, 1500), ba
     return params, cl_in, cl_out, report
 
 
+def split_closure(closure):
+    """(network (params, feature_scale, output_scale) or None, tau offset)."""
+    if closure is None:
+        return None, 0.0
+    return tuple(closure[:3]), (float(closure[3]) if len(closure) > 3 else 0.0)
+
+
 class CoarseRun:
-    """A coarse lattice with either closure."""
+    """A coarse lattice with either closure; a network closure's tau offset
+    shifts the lattice's base relaxation time."""
 
     def __init__(self, case: ClosureCase, nd_coarse, closure, storage="f32"):
         cx, cy, cz = case.coarse
@@ -369,16 +389,13 @@ class CoarseRun:
         self.nd = nd_coarse
         self.fluid = nd_coarse != geometry.SOLID
         _, prof = inlet_profiles(case)
-        params = None
-        cs = case.cs
-        if closure is not None:
-            params, cl_in, cl_out = closure
-            cs = 0.0
+        net, offset = split_closure(closure)
         self.lat = Lattice(19, cx, cy, cz, nd_coarse, storage=storage,
-                           omega=1.0 / case.tau_coarse, u_lbm=case.u_in, cs=cs)
+                           omega=1.0 / (case.tau_coarse + offset), u_lbm=case.u_in,
+                           cs=case.cs if net is None else 0.0)
         self.lat.set_inlet_profile(prof)
-        if params is not None:
-            self.lat.set_closure(params, cl_in, cl_out)
+        if net is not None:
+            self.lat.set_closure(*net)
 
     def start_from(self, u, rho):
         u = np.where(self.fluid, u, 0.0)
@@ -444,66 +461,100 @@ def short_horizon(case: ClosureCase, nd_c, data, closure, starts, horizons=(1, 4
 
 
 def statistics(case: ClosureCase, nd_c, data, closure, storage="f32", u_f=None):
-    """A coarse run from uniform flow over the fine run's time window
-    (acoustic scaling), sampled at the snapshot times: relative L2 of the
-    mean velocity and of the rms of the vortical fluctuation (deplane)
-    against the filtered fine run."""
+    """One coarse run from uniform flow over the fine run's time window
+    (acoustic scaling), sampled at the snapshot times.  Per window --
+    "train" (the training snapshots) and "heldout" (the rest) -- the
+    relative L2 of the mean velocity and of the rms of the vortical
+    fluctuation (deplane) against the filtered fine run over the same
+    snapshots."""
     fluid = nd_c == geometry.FLUID
     if u_f is None:
         u_f = np.stack(data["u"]).astype(np.float64)
-    u_mean = u_f.mean(axis=0)
-    v_f = np.stack([deplane(u, fluid, case) for u in u_f])
-    v_rms = v_f.std(axis=0)
+    ntr = case.snapshots - case.holdout
+    windows = {"train": (0, ntr), "heldout": (ntr, case.snapshots)}
     half = case.every // 2
+    acc = {w: [0.0, 0.0, 0.0] for w in windows}
     with CoarseRun(case, nd_c, closure, storage) as run:
         run.lat.init_from_macros(*uniform_start(nd_c, case.u_in))
         run.lat.step(case.spinup // 2)
-        s1 = np.zeros_like(u_mean)
-        v1 = np.zeros_like(u_mean)
-        v2 = np.zeros_like(u_mean)
-        for _ in range(case.snapshots):
+        for k in range(case.snapshots):
             run.lat.step(half)
             u = run.velocity()
             if not np.all(np.isfinite(u)):
-                return {"mean_l2": float("inf"), "rms_vort_l2": float("inf"),
-                        "rms_vort_ratio": float("inf"), "diverged": True}
+                bad = {"mean_l2": float("inf"), "rms_vort_l2": float("inf"),
+                       "rms_vort_ratio": float("inf"), "diverged": True}
+                return {w: dict(bad) for w in windows}
             v = deplane(u, fluid, case)
-            s1 += u
-            v1 += v
-            v2 += v * v
-    mean = s1 / case.snapshots
-    vm = v1 / case.snapshots
-    rms = np.sqrt(np.maximum(v2 / case.snapshots - vm * vm, 0.0))
-    return {"mean_l2": _l2(mean, u_mean, fluid), "rms_vort_l2": _l2(rms, v_rms, fluid),
-            "rms_vort_ratio": float(np.linalg.norm(rms[:, fluid]) /
-                                    np.linalg.norm(v_rms[:, fluid]))}
+            w = "train" if k < ntr else "heldout"
+            a = acc[w]
+            a[0] = a[0] + u
+            a[1] = a[1] + v
+            a[2] = a[2] + v * v
+    out = {}
+    for w, (j0, j1) in windows.items():
+        n = j1 - j0
+        if n == 0:
+            continue
+        uw = u_f[j0:j1]
+        v_rms = np.stack([deplane(u, fluid, case) for u in uw]).std(axis=0)
+        s1, v1, v2 = acc[w]
+        vm = v1 / n
+        rms = np.sqrt(np.maximum(v2 / n - vm * vm, 0.0))
+        out[w] = {"mean_l2": _l2(s1 / n, uw.mean(axis=0), fluid),
+                  "rms_vort_l2": _l2(rms, v_rms, fluid),
+                  "rms_vort_ratio": float(np.linalg.norm(rms[:, fluid]) /
+                                          np.linalg.norm(v_rms[:, fluid]))}
+    return out
 
 
-def calibrate(case: ClosureCase, nd_fine, data, closure, multipliers=(0.5, 1.0, 1.5, 2.0, 3.0),
-              starts=4, storage="f32"):
-    """A-posteriori choice of the output scale on TRAINING snapshots: the
-    multiplier of out_scale whose coarse runs stay closest (short-horizon
-    L2, mean over horizons) to the filtered fine run."""
+def _finite(x: float) -> float:
+    return x if np.isfinite(x) else float("inf")
+
+
+def calibrate(case: ClosureCase, nd_fine, data, closure, multipliers=(0.25, 0.5, 1.0),
+              offsets=(0.0, 0.004, 0.008, 0.012), starts=4, horizons=(1, 4, 8),
+              storage="f32"):
+    """A-posteriori choice of (m, c) on the TRAINING window: output_scale ->
+    m output_scale and base relaxation time tau0 + c.  Score = short-horizon
+    L2 (mean over horizons, runs started from training snapshots) over
+    Smagorinsky's + training-window mean-velocity L2 over Smagorinsky's;
+    runs that diverge are out.  Returns ((m, c), table)."""
     nd_c = coarse_geometry(nd_fine, case)
     u_f = np.stack(data["u"]).astype(np.float64)
     ntr = case.snapshots - case.holdout
-    js = list(np.linspace(0, ntr - 5, starts).astype(int))
-    params, cl_in, cl_out = closure
-    table = {}
+    js = [int(j) for j in np.linspace(0, ntr - 1 - max(horizons), starts).astype(int)]
+    net, _ = split_closure(closure)
+    params, cl_in, cl_out = net
+
+    def score(cl):
+        sh = short_horizon(case, nd_c, data, cl, js, horizons=horizons, storage=storage,
+                           u_f=u_f)
+        st = statistics(case, nd_c, data, cl, storage=storage, u_f=u_f)["train"]
+        return _finite(float(np.mean(list(sh.values())))), _finite(st["mean_l2"])
+
+    ref = score(None)
+    table = {"smagorinsky": {"short": ref[0], "mean": ref[1]}}
+    best, best_score = None, float("inf")
     for m in multipliers:
-        r = short_horizon(case, nd_c, data, (params, cl_in, cl_out * m), js, storage=storage,
-                          u_f=u_f)
-        table[str(m)] = float(np.mean(list(r.values())))
-    best = min(multipliers, key=lambda m: table[str(m)])
-    _log(f"calibration on training snapshots {js}: {table} -> x{best}")
+        for c in offsets:
+            sh, mean = score((params, cl_in, cl_out * m, c))
+            sc = sh / ref[0] + mean / ref[1]
+            table[f"{m}/{c}"] = {"short": sh, "mean": mean, "score": sc}
+            _log(f"calibration m {m} c {c}: short {sh:.4g} mean {mean:.4g} score {sc:.4g}")
+            if sc < best_score:
+                best, best_score = (m, c), sc
+    if best is None:
+        raise RuntimeError("every calibrated network run diverged")
+    _log(f"calibration on training window (starts {js}): smagorinsky short {ref[0]:.4g} "
+         f"mean {ref[1]:.4g} -> m {best[0]} c {best[1]} (score {best_score:.4g})")
     return best, table
 
 
 def evaluate(case: ClosureCase, nd_fine, data, closures: dict, *, starts=4,
              horizons=(1, 4, 8), storage="f32"):
-    """Scores each closure (name -> None for Smagorinsky or (params, cl_in,
-    cl_out)) against the filtered fine run: short horizons from held-out
-    snapshots, and the long-run statistics."""
+    """Scores each closure (name -> closure) against the filtered fine run:
+    short horizons from held-out snapshots, and the long-run statistics per
+    window."""
     nd_c = coarse_geometry(nd_fine, case)
     fluid = nd_c == geometry.FLUID
     u_f = np.stack(data["u"]).astype(np.float64)
@@ -511,7 +562,9 @@ def evaluate(case: ClosureCase, nd_fine, data, closures: dict, *, starts=4,
     js = list(range(first, case.snapshots - max(horizons)))[:starts]
     out = {"starts": [int(j) for j in js],
            "persistence": {str(h): float(np.mean([_l2(u_f[j], u_f[j + h], fluid) for j in js]))
-                           for h in horizons}}
+                           for h in horizons},
+           # how far the fine run's own mean moves between the two windows
+           "window_mean_shift": _l2(u_f[first:].mean(axis=0), u_f[:first].mean(axis=0), fluid)}
     for name, cl in closures.items():
         out[name] = {"short": short_horizon(case, nd_c, data, cl, js, horizons=horizons,
                                             storage=storage, u_f=u_f),
@@ -531,7 +584,7 @@ def step_cost(shape, closure, storage="f32", steps=200, warmup=20, repeats=3):
         with Lattice(19, nx, ny, nz, nd, storage=storage, omega=1.0 / 0.501, u_lbm=0.06,
                      cs=cs) as lat:
             if name == "network":
-                lat.set_closure(*closure)
+                lat.set_closure(*split_closure(closure)[0])
             lat.step(warmup)
             lat.sync()
             res[name] = min(lat.step_timed(steps) / steps for _ in range(repeats))
@@ -549,12 +602,11 @@ def run_pipeline(case: ClosureCase, *, train_steps=(2000, 1500, 1500), cost_shap
                             u_mean=u.mean(axis=0), u_rms=u.std(axis=0),
                             planes=np.array(FILTER_PLANES))
     params, cl_in, cl_out, rep = train_closure(case, data, steps=train_steps)
-    best, table = calibrate(case, nd, data, (params, cl_in, cl_out), storage=storage)
-    rep["calibration"] = {"multiplier": best, "train_short_l2": table}
-    cl_out *= best
-    closure = (params, cl_in, cl_out)
+    (m, c), table = calibrate(case, nd, data, (params, cl_in, cl_out), storage=storage)
+    rep["calibration"] = {"multiplier": m, "tau_offset": c, "table": table}
+    closure = (params, cl_in, cl_out * m, c)
     closures = {"smagorinsky": None, "network": closure,
-                "network_apriori": (params, cl_in, cl_out / best)}
+                "network_uncalibrated": (params, cl_in, cl_out)}
     ev = evaluate(case, nd, data, closures, storage=storage)
     cost = {}
     for shp in cost_shapes:
@@ -562,14 +614,15 @@ def run_pipeline(case: ClosureCase, *, train_steps=(2000, 1500, 1500), cost_shap
             cost[f"{shp[0]}x{shp[1]}x{shp[2]}_{st}"] = step_cost(shp, closure, storage=st)
     return {"case": dataclasses.asdict(case), "coarse": list(case.coarse),
             "tau_coarse": case.tau_coarse, "training": rep, "evaluation": ev, "cost": cost,
-            "params": params.tolist(), "feature_scale": cl_in, "output_scale": cl_out,
-            "seconds": time.time() - t0}
+            "params": params.tolist(), "feature_scale": cl_in, "output_scale": cl_out * m,
+            "tau_offset": c, "seconds": time.time() - t0}
 
 
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--scale", type=float, default=1.0, help="lattice size relative to 256x128x128")
-    ap.add_argument("--snapshots", type=int, default=50)
+    ap.add_argument("--snapshots", type=int, default=100,
+                    help="snapshots of the fine run; the second half is held out")
     ap.add_argument("--train-steps", type=int, nargs="+", default=[2000, 1500, 1500])
     ap.add_argument("--out", default=None, help="write the JSON report here")
     ap.add_argument("--cost", action="store_true", help="also time both closures")
@@ -588,14 +641,15 @@ def main(argv=None) -> int:
                 for st in ("f32", "f16")}
         print(json.dumps(cost, indent=1))
         return 0
-    case = scaled_case(a.scale, snapshots=a.snapshots) if a.scale != 1.0 else \
-        ClosureCase(snapshots=a.snapshots)
+    kw = {"snapshots": a.snapshots, "holdout": a.snapshots // 2}
+    case = scaled_case(a.scale, **kw) if a.scale != 1.0 else ClosureCase(**kw)
     shapes = [case.coarse, (128, 128, 512)] if a.cost else []
     rep = run_pipeline(case, train_steps=tuple(a.train_steps), cost_shapes=shapes,
                        dump=a.dump, dump_snapshots=a.dump_snapshots)
     text = json.dumps(rep)
     if a.save:
-        net = {k: rep[k] for k in ("params", "feature_scale", "output_scale", "case")}
+        net = {k: rep[k] for k in ("params", "feature_scale", "output_scale", "tau_offset",
+                                   "case")}
         net["calibration"] = rep["training"]["calibration"]
         net["apriori"] = rep["training"]["apriori"]
         with open(a.save, "w") as fh:
@@ -611,11 +665,12 @@ def main(argv=None) -> int:
 
 
 def load_network(path: str):
-    """(params, feature_scale, output_scale) of a network saved by --save."""
+    """(params, feature_scale, output_scale, tau_offset) of a network saved
+    by --save; the lattice runs at tau0 + tau_offset under it."""
     with open(path) as fh:
         net = json.load(fh)
     return (np.asarray(net["params"], np.float32), float(net["feature_scale"]),
-            float(net["output_scale"]))
+            float(net["output_scale"]), float(net.get("tau_offset", 0.0)))
 
 
 if __name__ == "__main__":

@@ -205,11 +205,13 @@ def _spheroid_run(dims, nd, body, storage, tau, steps, chunk, closure=None, cs=0
     """One run; drag = Fz on the body, final and averaged over `samples`
     instants evenly spread over the second half of the run (the wake is
     unsteady, so single instants of runs that drift apart differ)."""
+    from .closure import split_closure
     nx, ny, nz = dims
-    with Lattice(19, nx, ny, nz, nd, storage=storage, omega=1.0 / tau, u_lbm=0.05,
-                 cs=0.0 if closure is not None else cs) as lat:
-        if closure is not None:
-            lat.set_closure(*closure)
+    net, offset = split_closure(closure)
+    with Lattice(19, nx, ny, nz, nd, storage=storage, omega=1.0 / (tau + offset), u_lbm=0.05,
+                 cs=0.0 if net is not None else cs) as lat:
+        if net is not None:
+            lat.set_closure(*net)
         lat.init_from_macros(*uniform_start(nd, 0.05))
         tag = f"spheroid {nx}x{ny}x{nz} {storage}{'+network' if closure else ''}"
         ms = timed_run(lat, steps // 2, chunk, tag)
