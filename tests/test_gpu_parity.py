@@ -88,18 +88,23 @@ def test_fp64_random_state_matches_oracle(gpu_lib, geom, q, cs):
                 assert rel_err(fg[key], fo[key]) <= TOL64, key
 
 
+@pytest.mark.parametrize("addressing", ["relative", "generic"])
 @pytest.mark.parametrize("q", [15, 19, 27])
 @pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
-def test_relative_addressing_lattice_matches_oracle(gpu_lib, q, storage):
-    """An odd-sized lattice (33x17x40, walls, spheres, inlet/outlet) where
-    most workgroups take the 32-bit relative gather (planes 1..nz-2) and the
-    rest the general one, from a random state: FP64 to the FP64 bar, FP32
-    and FP16 (deviation storage) within their precision."""
+def test_relative_addressing_lattice_matches_oracle(gpu_lib, q, storage, addressing, monkeypatch):
+    """An odd-sized lattice (33x17x40, walls, spheres, inlet/outlet) from a
+    random state: FP64 to the FP64 bar, FP32 and FP16 (deviation storage)
+    within their precision.  "relative": most workgroups take the 32-bit
+    relative gather (planes 1..nz-2) and the rest the general one;
+    "generic": the general 64-bit gather everywhere, the path of lattices
+    too large for 32-bit relative offsets."""
+    if addressing == "generic":
+        monkeypatch.setenv("TLBM_GENERIC_ADDRESSING", "1")
     nx, ny, nz = 33, 17, 40
     nd = geometry.channel(nx, ny, nz, obstacles=[geometry.sphere(16, 8, 12, 5),
                                                  geometry.sphere(5, 9, 30, 3.5)])
     kw = dict(omega=1.6, u_lbm=0.04, rho_lbm=1.0, cs=0.125)
-    print(f"relative addressing q={q} {storage}", flush=True)
+    print(f"relative addressing q={q} {storage} {addressing}", flush=True)
     o = Oracle(q, nx, ny, nz, nd, **kw)
     f0 = _random_state(np.random.default_rng(q + 7), o.weights(), nx * ny * nz, amp=0.05)
     o.set_populations(f0)

@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -211,6 +212,10 @@ int tlbm_gpu_create(const tlbm_params *p, const uint8_t *nd_type, tlbm_gpu **out
         a.kb = uint32_t((2 * uint64_t(a.nxny) + CH - 1) / CH + 1);
         a.rel = (2 * uint64_t(a.kb) + 2) * CH * uint64_t(Q) * storage_bytes(p->storage) <
                 (uint64_t(1) << 32);
+        // TLBM_GENERIC_ADDRESSING=1: the 64-bit gather everywhere, the path
+        // of lattices too large for 32-bit relative offsets (for tests)
+        const char *gen = std::getenv("TLBM_GENERIC_ADDRESSING");
+        if (gen && gen[0] == '1') a.rel = 0;
     }
     switch (Q) {
     case 15: fill_weights<tlbm::D3Q15>(a, p->exact_weights != 0); break;
