@@ -12,18 +12,24 @@
 // w3 [32], b3 -- kMlpParams values.
 //
 // Inference runs on a whole wavefront: its 64 nodes form two blocks of 32
-// (lanes 0-31 and 32-63) and both hidden layers are 32x32x8 bf16 MFMA
-// (v_mfma_f32_32x32x8_bf16, FP32 accumulation) with the weights as the A
-// operand and 32 nodes as the columns:
-//   layer 1: H1^T = W1 (32x8, input 7 padded) x X^T (8 x 32 nodes), 1 MFMA,
-//   layer 2: H2^T = W2 (32x32) x H1^T, 4 MFMAs (K = 8 each).
-// In the MFMA layout lane l holds column l%32 and the rows
-// 8q + 4(l/32) + r (q, r = 0..3) of the output, which are exactly the K
-// rows layer 2 needs from lane l, so layer 1's output feeds layer 2 in
-// registers.  The only cross-lane moves are the 7 features (lane l <->
-// l^32) and the final sum of the two lane halves.  Weights, features and
-// h1 are rounded to bf16 (round to nearest even); biases, h2 and the last
-// layer are FP32.  The whole wavefront must execute mlp_wave together.
+// (lanes 0-31 and 32-63) and both hidden layers are bf16 MFMA with FP32
+// accumulation, the weights as the A operand and 32 nodes as the columns:
+//   layer 1: H1^T = W1 (32x8, input 7 padded) x X^T (8 x 32 nodes), one
+//            v_mfma_f32_32x32x8_bf16,
+//   layer 2: H2^T = W2 (32x32) x H1^T, two v_mfma_f32_32x32x16_bf16 (K = 16
+//            each; the full-rate bf16 instruction of gfx950).
+// In the 32x32 output layout lane l holds column l%32 and, in register
+// 4q + r, row 8q + 4(l/32) + r (q, r = 0..3).  A K = 16 MFMA multiplies
+// item i of lane group g (= l/32) of A with item i of lane group g of B,
+// so any assignment of the 16 K indices to (g, i) gives the same sum as
+// long as A and B agree on it: K block kb takes, as its 8 items in lane l,
+// layer 1's registers 8kb .. 8kb+7 (units 16kb + 4g + r and
+// 16kb + 8 + 4g + r, r = 0..3), and W2 is packed in that order.  So
+// layer 1's output feeds layer 2 in registers.  The only cross-lane moves
+// are the 7 features (lane l <-> l^32) and the final sum of the two lane
+// halves.  Weights, features and h1 are rounded to bf16 (round to nearest
+// even); biases, h2 and the last layer are FP32.  The whole wavefront must
+// execute mlp_wave together.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -38,7 +44,7 @@ constexpr int kMlpParams = kMlpH * kMlpIn + kMlpH + kMlpH * kMlpH + kMlpH + kMlp
 // Weights laid out per lane for the MFMA operands (built by mlp_pack).
 struct MlpPacked {
     uint32_t a1[64][2];     // W1[l%32][4(l/32)+t], t = 0..3, bf16 pairs
-    uint32_t a2[4][64][2];  // W2[l%32][8kb+4(l/32)+t]
+    uint32_t a2[2][64][4];  // W2[l%32][u], u = 16kb + 8(t/4) + 4(l/32) + t%4, t = 0..7
     float b1[2][16];        // b1 of unit 8q+4h+r at [h][4q+r]
     float b2[2][16];
     float w3[2][16];        // w3 rounded to bf16
@@ -46,6 +52,7 @@ struct MlpPacked {
 };
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __host__ __device__ inline uint16_t f32_to_bf16(float f) {
@@ -77,10 +84,10 @@ inline void mlp_pack(const float *p, MlpPacked *m) {
         }
         m->a1[l][0] = pair(v[0], v[1]);
         m->a1[l][1] = pair(v[2], v[3]);
-        for (int kb = 0; kb < 4; ++kb) {
-            for (int t = 0; t < 4; ++t) v[t] = W2[i * kMlpH + 8 * kb + 4 * h + t];
-            m->a2[kb][l][0] = pair(v[0], v[1]);
-            m->a2[kb][l][1] = pair(v[2], v[3]);
+        for (int kb = 0; kb < 2; ++kb) {
+            float u8[8];
+            for (int t = 0; t < 8; ++t) u8[t] = W2[i * kMlpH + 16 * kb + 8 * (t / 4) + 4 * h + t % 4];
+            for (int p2 = 0; p2 < 4; ++p2) m->a2[kb][l][p2] = pair(u8[2 * p2], u8[2 * p2 + 1]);
         }
     }
     for (int h = 0; h < 2; ++h)
@@ -128,9 +135,19 @@ __device__ inline uint32_t relu_bf16x2(uint32_t v) {
     return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(i16x2, v), z));
 }
 
-__device__ inline bf16x4 pack4_relu(float a, float b, float c, float d) {
-    const u32x2 w = {relu_bf16x2(cvt_pk_bf16(a, b)), relu_bf16x2(cvt_pk_bf16(c, d))};
-    return __builtin_bit_cast(bf16x4, w);
+__device__ inline bf16x8 pack8_relu(float a, float b, float c, float d, float e, float f,
+                                    float g, float h) {
+    const uint32_t w[4] = {relu_bf16x2(cvt_pk_bf16(a, b)), relu_bf16x2(cvt_pk_bf16(c, d)),
+                           relu_bf16x2(cvt_pk_bf16(e, f)), relu_bf16x2(cvt_pk_bf16(g, h))};
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 v = {w[0], w[1], w[2], w[3]};
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+__device__ inline bf16x8 load_quad(const uint32_t (&w)[4]) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 v = {w[0], w[1], w[2], w[3]};
+    return __builtin_bit_cast(bf16x8, v);
 }
 
 __device__ inline f32x16 load16(const float (&v)[16]) {
@@ -164,10 +181,11 @@ __device__ inline float mlp_wave(const float (&x)[kMlpIn], const MlpPacked *__re
         const f32x16 c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, xb[blk], load16(m->b1[h]), 0, 0, 0);
         f32x16 d = load16(m->b2[h]);
 #pragma unroll
-        for (int kb = 0; kb < 4; ++kb) {
-            // register 4kb+t holds unit 8kb+4h+t: the K rows of block kb
-            const bf16x4 hk = pack4_relu(c[4 * kb], c[4 * kb + 1], c[4 * kb + 2], c[4 * kb + 3]);
-            d = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(load_pair(m->a2[kb][l]), hk, d, 0, 0, 0);
+        for (int kb = 0; kb < 2; ++kb) {
+            // registers 8kb .. 8kb+7: the 8 items of K block kb in this lane
+            const bf16x8 hk = pack8_relu(c[8 * kb], c[8 * kb + 1], c[8 * kb + 2], c[8 * kb + 3],
+                                         c[8 * kb + 4], c[8 * kb + 5], c[8 * kb + 6], c[8 * kb + 7]);
+            d = __builtin_amdgcn_mfma_f32_32x32x16_bf16(load_quad(m->a2[kb][l]), hk, d, 0, 0, 0);
         }
         float p = 0.f;
 #pragma unroll
