@@ -682,7 +682,11 @@ k_lbm_closure(const typename StorageT<ST>::st *__restrict__ fin, typename Storag
     float feat[kMlpIn];
 #pragma unroll
     for (int k = 0; k < 6; ++k) feat[k] = float(pi[k]) * a.cl_in;
-    feat[6] = float(stress_norm<T>(pi)) * a.cl_in;
+    // the norm feature is rounded to bf16: the hardware square root (1 ulp) suffices
+    feat[6] = __builtin_amdgcn_sqrtf(float(pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +
+                                           T(2) * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5]))) *
+              a.cl_in;
+
     __syncthreads();  // smlp complete (every lane of the workgroup gets here)
     const float zz = mlp_wave(feat, &smlp);
     const float om = __builtin_amdgcn_rcpf(a.tau0 + a.cl_out * softplus_f(zz));  // 1 ulp

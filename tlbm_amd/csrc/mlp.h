@@ -197,8 +197,13 @@ __device__ inline float mlp_wave(const float (&x)[kMlpIn], const MlpPacked *__re
     return (h ? pb : pa) + m->b3;
 }
 
-// softplus with the hardware exp/log (a value below float resolution of
-// 1 + e^z is dropped: it moves the relaxation time by less than 1e-7).
-__device__ inline float softplus_f(float z) { return z > 20.f ? z : __logf(1.f + __expf(z)); }
+// softplus with the hardware exp2/log2 (v_exp_f32, v_log_f32, 1 ulp each):
+// the argument of the logarithm is at least 1, so neither needs range
+// handling; a value below float resolution of 1 + e^z is dropped (it moves
+// the relaxation time by less than 1e-7).
+__device__ inline float softplus_f(float z) {
+    return z > 20.f ? z
+                    : __builtin_amdgcn_logf(1.f + __builtin_amdgcn_exp2f(z * 1.44269504f)) * 0.693147181f;
+}
 
 }  // namespace tlbm
