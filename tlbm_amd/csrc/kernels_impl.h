@@ -22,7 +22,7 @@ struct Launch {
         st *out = static_cast<st *>(fout);
         void *mac = macro;
         const T *uyh = static_cast<const T *>(uy_head);
-        using TU = Tune<ST>;
+        using TU = Tune<ST, L::Q>;
         switch (mode) {
         case MODE_STEP:
             if (a.closure) {

@@ -147,24 +147,30 @@ template <> struct StorageT<ST_F16> {
     }
 };
 
-// Launch choices per storage, measured on MI355X with the D3Q19
-// 128x128x512 lattice: one node per lane everywhere, FP64 in 1024-thread
-// workgroups with non-temporal loads (ahead of a two-nodes-per-lane 16-byte
-// variant: 20.9 against 19.8 GLUPS, and 13.5 against 12.0 for D3Q27; 1024
-// threads ahead of 512), FP32 in 256-thread workgroups (203 against 205 us
-// with 512), FP16 in 512-thread workgroups, both with plain loads (FP16:
-// 109.7 against 111.5 us per step with non-temporal loads).  Every kernel
-// stores non-temporally.
-template <int ST> struct Tune;
-template <> struct Tune<ST_F64> {
-    static constexpr int kBS = 1024;
-    static constexpr bool kNTLoad = true;
+// Launch choices per storage and lattice, measured on MI355X: one node per
+// lane everywhere.
+// * FP64 D3Q19 (128x128x512): 1024-thread workgroups with non-temporal
+//   loads, ahead of 512 threads and of a two-nodes-per-lane 16-byte
+//   variant (20.9 against 19.8 GLUPS, and 13.5 against 12.0 for D3Q27).
+// * FP64 D3Q27 (porous 384^3, 128 VGPRs): 512-thread workgroups with plain
+//   loads, 4.12 ms per step against 4.38 for 1024 threads with
+//   non-temporal loads (4.16 for 512 with them, 4.53 for 256).
+// * FP32: 256-thread workgroups (D3Q19: 203 against 205 us with 512; D3Q27
+//   porous: 2.05 ms, the same as 512 and 1024).
+// * FP16: 512-thread workgroups.
+// FP32 and FP16 use plain loads: FP16 109.7 against 111.5 us per step with
+// non-temporal loads; FP32 D3Q27 2.05 against 2.17 ms.  Every kernel stores
+// non-temporally.
+template <int ST, int Q> struct Tune;
+template <int Q> struct Tune<ST_F64, Q> {
+    static constexpr int kBS = Q == 27 ? 512 : 1024;
+    static constexpr bool kNTLoad = Q != 27;
 };
-template <> struct Tune<ST_F32> {
+template <int Q> struct Tune<ST_F32, Q> {
     static constexpr int kBS = 256;
     static constexpr bool kNTLoad = false;
 };
-template <> struct Tune<ST_F16> {
+template <int Q> struct Tune<ST_F16, Q> {
     static constexpr int kBS = 512;
     static constexpr bool kNTLoad = false;
 };
