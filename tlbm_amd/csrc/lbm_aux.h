@@ -199,9 +199,38 @@ k_box_filter(const T *__restrict__ macro, double *__restrict__ out, StepArgs a) 
     for (int k = 0; k < kFilterPlanes; ++k) out[size_t(k) * Nc + c] = acc[k] * 0.125;
 }
 
+// Sum over the 64 lanes of a wavefront: a fixed-order butterfly of lane
+// exchanges (no LDS); lane 0 holds the result.
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// The kRed sums of a kBlock-thread workgroup: wavefront sums, then the
+// kBlock/64 wave totals folded in wave order by threads 0..kRed-1, which
+// return them (other threads return zeros).  Deterministic.
+__device__ inline void block_sums(double (&acc)[kRed]) {
+    constexpr int W = kBlock / 64;
+    __shared__ double sh[kRed][W];
+    const int lane = int(threadIdx.x & 63u), wave = int(threadIdx.x >> 6);
+#pragma unroll
+    for (int k = 0; k < kRed; ++k) {
+        const double v = wave_sum(acc[k]);
+        if (lane == 0) sh[k][wave] = v;
+    }
+    __syncthreads();
+    const int k = int(threadIdx.x);
+    double t = 0.0;
+    if (k < kRed) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) t += sh[k][w];
+    }
+    acc[0] = t;
+}
+
 // Per-workgroup FP64 partial sums over the state: sum f, sum f e_x,
 // sum f e_y, sum f e_z, sum (B_n - B_{n-1})^2, sum B_n^2 (kRed values).
-
 template <class L, int ST>
 __global__ void __launch_bounds__(kBlock)
 k_reduce_partial(const typename StorageT<ST>::st *__restrict__ cur,
@@ -224,18 +253,8 @@ k_reduce_partial(const typename StorageT<ST>::st *__restrict__ cur,
             }
         }
     }
-    __shared__ double sh[kRed][kBlock];
-#pragma unroll
-    for (int k = 0; k < kRed; ++k) sh[k][threadIdx.x] = acc[k];
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (int(threadIdx.x) < w) {
-#pragma unroll
-            for (int k = 0; k < kRed; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x < kRed) partial[size_t(blockIdx.x) * kRed + threadIdx.x] = sh[threadIdx.x][0];
+    block_sums(acc);
+    if (threadIdx.x < kRed) partial[size_t(blockIdx.x) * kRed + threadIdx.x] = acc[0];
 }
 
 // One workgroup folds the partials in a fixed order (deterministic).
@@ -243,23 +262,13 @@ k_reduce_partial(const typename StorageT<ST>::st *__restrict__ cur,
 template <int ST>
 __global__ void __launch_bounds__(kBlock)
 k_reduce_final(const double *__restrict__ partial, int nparts, double *__restrict__ out) {
-    __shared__ double sh[kRed][kBlock];
     double acc[kRed] = {0, 0, 0, 0, 0, 0};
     for (int i = threadIdx.x; i < nparts; i += kBlock) {
 #pragma unroll
         for (int k = 0; k < kRed; ++k) acc[k] += partial[size_t(i) * kRed + k];
     }
-#pragma unroll
-    for (int k = 0; k < kRed; ++k) sh[k][threadIdx.x] = acc[k];
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (int(threadIdx.x) < w) {
-#pragma unroll
-            for (int k = 0; k < kRed; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x < kRed) out[threadIdx.x] = sh[threadIdx.x][0];
+    block_sums(acc);
+    if (threadIdx.x < kRed) out[threadIdx.x] = acc[0];
 }
 
 }  // namespace tlbm
