@@ -94,18 +94,19 @@ def _channel():
     return nx, ny, nz, nd
 
 
+@pytest.mark.parametrize("q", [15, 19, 27])
 @pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_constant_network_closure_equals_bgk(gpu_lib, storage):
+def test_constant_network_closure_equals_bgk(gpu_lib, storage, q):
     nx, ny, nz, nd = _channel()
     omega, c, cl_out = 1.6, 0.3, 0.05
     params = np.zeros(MLP_NPARAM, np.float32)
     params[-1] = c
-    with Lattice(19, nx, ny, nz, nd, storage=storage, omega=omega, u_lbm=0.05) as a:
+    with Lattice(q, nx, ny, nz, nd, storage=storage, omega=omega, u_lbm=0.05) as a:
         a.set_closure(params, 1.0, cl_out)
         a.step(60)
         ma = a.macros()
     om2 = 1.0 / (1.0 / omega + cl_out * np.log1p(np.exp(c)))
-    with Lattice(19, nx, ny, nz, nd, storage=storage, omega=om2, u_lbm=0.05) as b:
+    with Lattice(q, nx, ny, nz, nd, storage=storage, omega=om2, u_lbm=0.05) as b:
         b.step(60)
         mb = b.macros()
     fluid = nd == 0
@@ -117,14 +118,22 @@ def test_constant_network_closure_equals_bgk(gpu_lib, storage):
     assert np.linalg.norm(u_a - u_b) / np.linalg.norm(u_b) < tol
 
 
-def test_closure_run_follows_oracle(gpu_lib):
+@pytest.mark.parametrize("q", [15, 19, 27])
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_closure_run_follows_oracle(gpu_lib, storage, q):
+    """The network closure in the collide kernel against the oracle's FP64
+    closure run (bf16 network roundings included) on a 16x12x24 channel
+    whose interior workgroups take the relative gather; FP16 storage adds
+    its own rounding (~1e-4 per step)."""
     nx, ny, nz, nd = _channel()
     params = mlp_init(3)
     params[-1] = -1.0
     kw = dict(omega=1.0 / 0.51, u_lbm=0.06, rho_lbm=1.0)
-    o = Oracle(19, nx, ny, nz, nd, **kw)
+    o = Oracle(q, nx, ny, nz, nd, **kw)
     o.set_closure(params, 40.0, 0.02)
-    with Lattice(19, nx, ny, nz, nd, storage="f32", **kw) as lat:
+    tol = 1e-4 if storage == "f32" else 2e-3
+    print(f"closure run q={q} {storage}", flush=True)
+    with Lattice(q, nx, ny, nz, nd, storage=storage, **kw) as lat:
         lat.set_closure(params, 40.0, 0.02)
         for n in (2, 30):
             lat.step(n)
@@ -133,7 +142,7 @@ def test_closure_run_follows_oracle(gpu_lib):
             fluid = nd != 1  # after one step from rest only inlet/outlet nodes move
             u = np.stack([m[k][fluid] for k in ("ux", "uy", "uz")])
             ur = np.stack([r[k][fluid] for k in ("ux", "uy", "uz")])
-            assert np.linalg.norm(u - ur) / np.linalg.norm(ur) < 1e-4, n
+            assert np.linalg.norm(u - ur) / np.linalg.norm(ur) < tol, n
 
 
 def _host_stress(A, e, w, omega, cs):
