@@ -145,6 +145,32 @@ def test_closure_run_follows_oracle(gpu_lib, storage, q):
             assert np.linalg.norm(u - ur) / np.linalg.norm(ur) < tol, n
 
 
+def test_set_closure_rejects_bad_input(gpu_lib):
+    """FP64 storage, non-positive or non-finite scales and non-finite
+    parameters are refused with an error; the lattice keeps running on its
+    own rule and accepts a valid network afterwards."""
+    from tlbm_amd.gpu import TlbmError
+    nx, ny, nz, nd = _channel()
+    params = mlp_init(3)
+    with Lattice(19, nx, ny, nz, nd, storage="f64", omega=1.6, u_lbm=0.05) as a:
+        with pytest.raises(TlbmError, match="FP32 or FP16"):
+            a.set_closure(params, 1.0, 0.01)
+    with Lattice(19, nx, ny, nz, nd, storage="f32", omega=1.6, u_lbm=0.05) as b:
+        for fs, os_ in ((0.0, 0.01), (1.0, -0.01), (np.inf, 0.01), (1.0, np.nan)):
+            with pytest.raises(TlbmError, match="scales"):
+                b.set_closure(params, fs, os_)
+        bad = params.copy()
+        bad[17] = np.nan
+        with pytest.raises(TlbmError, match="finite"):
+            b.set_closure(bad, 1.0, 0.01)
+        with pytest.raises(ValueError):
+            b.set_closure(params[:-1], 1.0, 0.01)
+        b.step(5)
+        b.set_closure(params, 1.0, 0.01)
+        b.step(5)
+        assert np.all(np.isfinite(b.macros()["uz"]))
+
+
 def _host_stress(A, e, w, omega, cs):
     """Viscous stress -(1 - omega'/2) Pi_neq of pre-collision populations
     A (N, Q) on fluid nodes, with the Smagorinsky omega' of

@@ -504,8 +504,9 @@ int tlbm_gpu_set_closure(tlbm_gpu *h, const float *params, double feature_scale,
     }
     if (h->p.storage == TLBM_STORAGE_F64)
         return fail(TLBM_EINVAL, "the learned closure runs with FP32 or FP16 storage");
-    if (!(feature_scale > 0.0) || !(output_scale > 0.0))
-        return fail(TLBM_EINVAL, "closure scales must be positive");
+    if (!(feature_scale > 0.0) || !(output_scale > 0.0) || !std::isfinite(feature_scale) ||
+        !std::isfinite(output_scale))
+        return fail(TLBM_EINVAL, "closure scales must be positive and finite");
     for (int k = 0; k < tlbm::kMlpParams; ++k)
         if (!std::isfinite(params[k])) return fail(TLBM_EINVAL, "closure parameters must be finite");
     tlbm::MlpPacked packed;
