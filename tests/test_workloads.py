@@ -58,3 +58,27 @@ def test_spheroid_runner_small(gpu_lib):
     down = rep["downsampled"]
     assert down["f32_smagorinsky"]["l2_velocity_vs_f64"] < 1e-4
     assert down["f16_smagorinsky"]["l2_velocity_vs_f64"] < 1e-2
+
+
+def test_slab_case_geometry():
+    """The 8-way slab of the quarter-scale spheroid lattice: 256x128x16,
+    planes 48..63 of the full lattice, walls and the body's cross-sections
+    solid, no inlet or outlet nodes."""
+    (nx, ny, nz), nd, z0 = workloads.slab_case(8, scale=4)
+    assert (nx, ny, nz) == (256, 128, 16) and z0 == 48
+    v = nd.reshape(nz, ny, nx)
+    assert set(np.unique(nd)) == {geometry.FLUID, geometry.SOLID}
+    assert np.all(v[:, :, 0] == geometry.SOLID) and np.all(v[:, 0, :] == geometry.SOLID)
+    full = workloads.spheroid_mask(nx, ny, 128, nx / 2, ny / 2, 64, 8.0, 24.0)
+    inner = v[:, 1:-1, 1:-1] == geometry.SOLID
+    assert np.array_equal(inner, full[z0:z0 + nz, 1:-1, 1:-1])
+    assert inner.any()
+
+
+@pytest.mark.gpu
+def test_slab_runner_small(gpu_lib):
+    rep = workloads.run_slab(gpus=8, steps=20, chunk=10, scale=16)
+    assert rep["lattice"] == [64, 32, 4]
+    for r in rep["runs"].values():
+        assert r["mlups"] > 0 and r["face_exchange_us"] > 0
+

@@ -23,6 +23,13 @@
   inlet-outlet channel's acoustic mode loads the body with a force that
   swings between +130 and -35 over the run.
 
+* slab: one GPU's share of the spheroid lattice cut into z-slabs (1024 x
+  512 x 64 for 8 GPUs, the slab below the body's centre plane: walls and the
+  body's mid-section, no inlet or outlet), periodic in z -- a ring of one,
+  whose own z wrap stands in for the ghost planes -- timed for FP16 +
+  network and FP32 Smagorinsky beside the face exchange a sharded step would
+  overlap (the 5 crossing D3Q19 directions of one face over one xGMI link).
+
 Every run is timed with HIP events around batches of steps (kernel time on
 the lattice's stream), so MLUPS = nodes x steps / time, counting every node
 as the reference does (tLBM.cpp:71-72), and HBM GB/s = MLUPS x 2 Q bytes per
@@ -36,6 +43,7 @@ averaged over the second half of the run.
 
     python -m tlbm_amd.workloads porous  [--n 384] [--steps 5000]
     python -m tlbm_amd.workloads spheroid [--scale 1] [--steps 10000] [--network FILE]
+    python -m tlbm_amd.workloads slab [--gpus 8] [--steps 1000]
 """
 from __future__ import annotations
 
@@ -269,6 +277,62 @@ def run_spheroid(scale: int = 1, steps: int = 10000, chunk: int = 1000,
     return out
 
 
+def slab_case(gpus: int = 8, scale: int = 1):
+    """One GPU's share of the spheroid lattice cut into `gpus` z-slabs: the
+    slab just below the body's centre plane, (1024, 512, 512/gpus) at full
+    scale, side walls and the body solid, no inlet or outlet (an interior
+    slab).  Returns (dims, nd, z0)."""
+    nx, ny, nz = 1024 // scale, 512 // scale, 512 // scale
+    if nz % gpus:
+        raise ValueError(f"{nz} planes do not split into {gpus} slabs")
+    nzs = nz // gpus
+    z0 = (gpus // 2 - 1) * nzs if gpus > 1 else 0
+    body = spheroid_mask(nx, ny, nzs, nx / 2, ny / 2, nz / 2 - z0, 32 / scale, 96 / scale)
+    nd = np.zeros((nzs, ny, nx), np.uint8)
+    nd[body] = geometry.SOLID
+    nd[:, :, 0] = nd[:, :, nx - 1] = geometry.SOLID
+    nd[:, 0, :] = nd[:, ny - 1, :] = geometry.SOLID
+    return (nx, ny, nzs), nd.ravel(), z0
+
+
+def run_slab(gpus: int = 8, steps: int = 1000, chunk: int = 500, scale: int = 1,
+             network: str = DEFAULT_NETWORK, link_gbs: float = 153.0):
+    """The kernel on one GPU's share of the z-sharded spheroid lattice, at
+    its real shape.  The slab is periodic in z (a ring of one: the step's
+    own z wrap fills the ghost planes locally), which is the work each GPU
+    does per step.  It is timed for FP16 + network and FP32 Smagorinsky and
+    set beside the face exchange the sharded step would have to hide: the
+    populations crossing a face (the 5 D3Q19 directions with e_z = +1, or
+    -1) of nx ny nodes over one xGMI link at `link_gbs`."""
+    from .closure import load_network, split_closure
+    dims, nd, z0 = slab_case(gpus, scale)
+    nx, ny, nzs = dims
+    net, offset = split_closure(load_network(network))
+    out = {"workload": "spheroid_slab_d3q19", "lattice": list(dims), "slabs": gpus,
+           "z_range": [z0, z0 + nzs], "steps": steps, "runs": {}}
+    zero = np.zeros(nd.size)
+    uz = np.where(nd != geometry.SOLID, 0.05, 0.0)
+    for storage, cl in (("f16", net), ("f32", None)):
+        tau = 0.51 + (offset if cl is not None else 0.0)
+        with Lattice(19, nx, ny, nzs, nd, storage=storage, omega=1.0 / tau, u_lbm=0.05,
+                     cs=0.0 if cl is not None else 0.1) as lat:
+            if cl is not None:
+                lat.set_closure(*cl)
+            lat.init_from_macros(zero, zero, uz, np.ones(nd.size))
+            lat.step(20)
+            tag = f"slab {nx}x{ny}x{nzs} {storage}{'+network' if cl is not None else ''}"
+            ms = timed_run(lat, steps, chunk, tag)
+        r = _rate(nd.size, 19, storage, steps, ms)
+        face = 5 * nx * ny * STORAGE_BYTES[storage]
+        r["face_bytes"] = face
+        r["face_exchange_us"] = face / (link_gbs * 1e3)
+        # the step's interior planes: the window the face exchange overlaps with
+        r["interior_us"] = r["ms_per_step"] * 1e3 * (nzs - 2) / nzs
+        out["runs"][f"{storage}_{'network' if cl is not None else 'smagorinsky'}"] = r
+        _log(f"{tag}: {r}")
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description="measured workloads")
     sub = ap.add_subparsers(dest="which", required=True)
@@ -283,11 +347,18 @@ def main(argv=None) -> int:
     s.add_argument("--steps", type=int, default=10000)
     s.add_argument("--network", default=DEFAULT_NETWORK)
     s.add_argument("--no-downsampled", action="store_true")
-    for q in (p, s):
+    b = sub.add_parser("slab", help="one GPU's share of the spheroid lattice split into z-slabs")
+    b.add_argument("--gpus", type=int, default=8)
+    b.add_argument("--steps", type=int, default=1000)
+    b.add_argument("--scale", type=int, default=1)
+    b.add_argument("--network", default=DEFAULT_NETWORK)
+    for q in (p, s, b):
         q.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     if a.which == "porous":
         rep = run_porous(a.n, a.steps, seed=a.seed, ramp=a.ramp)
+    elif a.which == "slab":
+        rep = run_slab(a.gpus, a.steps, scale=a.scale, network=a.network)
     else:
         rep = run_spheroid(a.scale, a.steps, network=a.network,
                            compare_down=not a.no_downsampled)
