@@ -15,6 +15,7 @@ import numpy as np
 import pytest
 
 import goldens
+import lattice_tables
 from pyoracle import Oracle
 from tlbm_amd import geometry
 from tlbm_amd.gpu import Lattice
@@ -279,7 +280,8 @@ def test_fp32_matches_oracle_fp64_on_config_channel(gpu_lib):
 
 def test_translation_invariance_periodic_box(gpu_lib):
     """Shifting state and geometry periodically commutes with the step
-    (size-independent; exact in every storage)."""
+    (size-independent): exactly in FP64 and FP32; in FP16 up to its
+    stochastic rounding, whose random bits follow the node index."""
     nx, ny, nz, q = 32, 24, 40, 19
     nd = geometry.periodic_box(nx, ny, nz, obstacles=[geometry.sphere(9, 7, 11, 4.5)])
     rng = np.random.default_rng(2)
@@ -299,4 +301,47 @@ def test_translation_invariance_periodic_box(gpu_lib):
                 lat.set_populations(roll(f0) if rolled else f0)
                 lat.step(5)
                 outs.append(lat.populations())
-        assert np.array_equal(roll(outs[0]), outs[1]), storage
+        if storage == "f16":
+            w = o.weights()
+            d0, d1 = (roll(outs[0]) - w) / w, (outs[1] - w) / w
+            assert np.max(np.abs(d0 - d1)) <= 4 * 2.0 ** -11 * (1.0 + np.max(np.abs(d1)))
+            assert not np.array_equal(d0, d1)
+        else:
+            assert np.array_equal(roll(outs[0]), outs[1]), storage
+
+
+def test_fp16_step_rounds_stochastically(gpu_lib):
+    """The FP16 step stores its results with stochastic rounding.
+
+    A 16^3 periodic box of uniform flow (u_z 0.04) starts from an
+    equilibrium state whose scaled deviations (f - w)/w are FP16 values,
+    and takes one step in FP16 and in FP32 storage.  Every node computes the
+    same value; FP32 keeps it, FP16 rounds it up or down.  Per direction the
+    FP16 results average to the FP32 value within 0.05 FP16 ulp (the mean of
+    4096 unbiased roundings has a standard deviation below 0.008 ulp; round
+    to nearest would be off by its fixed rounding error, up to 0.5 ulp), and
+    they differ between nodes."""
+    n, q = 16, 19
+    nd = geometry.periodic_box(n, n, n)
+    e, _, w, _ = lattice_tables.tables(q)
+    u = np.array([0.0, 0.0, 0.04])
+    cu = 3.0 * (e @ u)
+    feq = w * (1.0 + cu + 0.5 * cu * cu - 1.5 * (u @ u))
+    h16 = ((feq - w) / w).astype(np.float16).astype(np.float64)
+    f0 = np.tile(w * (1.0 + h16), (n ** 3, 1))
+    out = {}
+    for storage in ("f16", "f32"):
+        with Lattice(q, n, n, n, nd, storage=storage, omega=1.6, u_lbm=0.0) as lat:
+            lat.set_populations(f0)
+            lat.step(1)
+            out[storage] = lat.populations()
+    ref = out["f32"].mean(axis=0)
+    assert np.all(out["f32"].std(axis=0) <= 1e-7 * np.abs(ref))
+    spread = 0
+    for s in range(q):
+        hs = (out["f16"][:, s] - w[s]) / w[s]
+        ulp = float(np.spacing(np.float16(abs((ref[s] - w[s]) / w[s]))))
+        bias = (hs.mean() - (ref[s] - w[s]) / w[s]) / ulp
+        assert abs(bias) < 0.05, (s, bias)
+        spread += len(np.unique(hs)) > 1
+    assert spread >= q - 1

@@ -82,3 +82,13 @@ def test_slab_runner_small(gpu_lib):
     for r in rep["runs"].values():
         assert r["mlups"] > 0 and r["face_exchange_us"] > 0
 
+
+
+@pytest.mark.gpu
+def test_cylinders_runner_short(gpu_lib):
+    rep = workloads.run_cylinders(steps=20, chunk=10, checkpoints=(10, 20))
+    assert [c["step"] for c in rep["checkpoints"]] == [10, 20]
+    last = rep["checkpoints"][-1]
+    assert last["l2_velocity_f32_vs_f64"] < 1e-5
+    assert last["l2_velocity_f16_vs_f64"] < 1e-3
+    assert set(rep["runs"]) == {"f64", "f32", "f16"}

@@ -104,6 +104,7 @@ struct tlbm_gpu {
     int nparts = 0;
     bool has_lid = false;
     long long steps_since_set = 0;
+    uint64_t step_count = 0;  // steps since creation (FP16 rounding stream)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t bytes = 0;
@@ -310,6 +311,7 @@ static int queue_steps(tlbm_gpu *h, int nsteps) {
                                        h->uy_head, nullptr, h->args, h->stream);
             if (e != hipSuccess) return fail(TLBM_EHIP, std::string("type-5 helper: ") + hipGetErrorString(e));
         }
+        h->args.step = uint32_t(h->step_count++);
         hipError_t e = h->ops->lbm(h->Q, tlbm::MODE_STEP, fin, fout, h->nd, h->chunk_type, nullptr,
                                    h->uy_head, h->args, h->stream);
         if (e != hipSuccess) return fail(TLBM_EHIP, std::string("stream-collide: ") + hipGetErrorString(e));

@@ -23,6 +23,9 @@
   inlet-outlet channel's acoustic mode loads the body with a force that
   swings between +130 and -35 over the run.
 
+* cylinders: the stream-collide benchmark's lattice (D3Q19 BGK 128x128x512,
+  cylinder array, inlet u_z 0.04, tau 0.55) over the configuration's full
+  20000 steps from rest, FP32 and FP16 against FP64 at checkpoints.
 * slab: one GPU's share of the spheroid lattice cut into z-slabs (1024 x
   512 x 64 for 8 GPUs, the slab below the body's centre plane: walls and the
   body's mid-section, no inlet or outlet), periodic in z -- a ring of one,
@@ -43,6 +46,7 @@ averaged over the second half of the run.
 
     python -m tlbm_amd.workloads porous  [--n 384] [--steps 5000]
     python -m tlbm_amd.workloads spheroid [--scale 1] [--steps 10000] [--network FILE]
+    python -m tlbm_amd.workloads cylinders [--steps 20000]
     python -m tlbm_amd.workloads slab [--gpus 8] [--steps 1000]
 """
 from __future__ import annotations
@@ -202,6 +206,52 @@ def run_porous(n: int = 384, steps: int = 5000, chunk: int = 500, seed: int = 20
 # ---------------------------------------------------------------------------
 
 
+def run_cylinders(steps: int = 20000, chunk: int = 500,
+                  checkpoints=(1100, 2000, 5000, 10000, 20000)):
+    """The stream-collide benchmark's lattice over the configuration's full
+    length: D3Q19 BGK 128x128x512, periodic x and y, a cylinder along x
+    (r 12 at (y, z) = (64, 128)), Zou-He inlet u_z 0.04 and outlet rho 1,
+    tau 0.55, from rest.  FP64, FP32 and FP16 lattices advance side by side;
+    the velocity L2 and mass drift of FP32 and FP16 against FP64, and the
+    cylinder's drag (z momentum exchange) in each, are recorded at the
+    checkpoints."""
+    nx, ny, nz = 128, 128, 512
+    nd = geometry.channel(nx, ny, nz, walls=(), obstacles=[geometry.cylinder_x(64, 128, 12)])
+    N, Q = nd.size, 19
+    fluid = nd != geometry.SOLID
+    body = nd == geometry.SOLID
+    out = {"workload": "cylinder_array_d3q19_128x128x512", "steps": steps, "runs": {},
+           "checkpoints": []}
+    kw = dict(omega=1.0 / 0.55, u_lbm=0.04)
+    lats = {st: Lattice(19, nx, ny, nz, nd, storage=st, **kw) for st in ("f64", "f32", "f16")}
+    ms = {st: 0.0 for st in lats}
+    done = 0
+    try:
+        while done < steps:
+            nxt = min([c for c in checkpoints if c > done] + [steps, done + chunk])
+            for st, lat in lats.items():
+                ms[st] += lat.step_timed(nxt - done)
+            done = nxt
+            if done in checkpoints or done == steps:
+                m = {st: lat.macros() for st, lat in lats.items()}
+                mass = {st: lat.reduce()["mass"] for st, lat in lats.items()}
+                drag = {st: float(np.sum(lat.forces()["Fz"][body])) for st, lat in lats.items()}
+                cp = {"step": done, "drag": drag}
+                for st in ("f32", "f16"):
+                    cp[f"l2_velocity_{st}_vs_f64"] = _l2(m[st], m["f64"], fluid)
+                    cp[f"mass_drift_{st}_vs_f64"] = abs(mass[st] - mass["f64"]) / mass["f64"]
+                out["checkpoints"].append(cp)
+                _log(f"cylinders step {done}: {cp}")
+                del m
+        for st, lat in lats.items():
+            out["runs"][st] = _rate(N, Q, st, steps, ms[st])
+    finally:
+        for lat in lats.values():
+            lat.close()
+    _log(f"cylinders runs: {out['runs']}")
+    return out
+
+
 def spheroid_case(scale: int = 1):
     """(nd, spheroid mask flat) of the spheroid channel at 1/scale size."""
     nx, ny, nz = 1024 // scale, 512 // scale, 512 // scale
@@ -347,16 +397,20 @@ def main(argv=None) -> int:
     s.add_argument("--steps", type=int, default=10000)
     s.add_argument("--network", default=DEFAULT_NETWORK)
     s.add_argument("--no-downsampled", action="store_true")
+    c = sub.add_parser("cylinders", help="the benchmark lattice over its full 20000 steps")
+    c.add_argument("--steps", type=int, default=20000)
     b = sub.add_parser("slab", help="one GPU's share of the spheroid lattice split into z-slabs")
     b.add_argument("--gpus", type=int, default=8)
     b.add_argument("--steps", type=int, default=1000)
     b.add_argument("--scale", type=int, default=1)
     b.add_argument("--network", default=DEFAULT_NETWORK)
-    for q in (p, s, b):
+    for q in (p, s, b, c):
         q.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     if a.which == "porous":
         rep = run_porous(a.n, a.steps, seed=a.seed, ramp=a.ramp)
+    elif a.which == "cylinders":
+        rep = run_cylinders(a.steps)
     elif a.which == "slab":
         rep = run_slab(a.gpus, a.steps, scale=a.scale, network=a.network)
     else:
