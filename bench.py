@@ -173,7 +173,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cpu-cores", type=int, default=16)
     ap.add_argument("--cpu-timeout", type=float, default=240.0)
-    ap.add_argument("--storages", default="f32,f64,f16",
+    ap.add_argument("--storages", default="f32,f64,f16,f32x",
                     help="storages to run; the first is the headline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (counter calibration)")

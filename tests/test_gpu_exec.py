@@ -45,15 +45,16 @@ def test_exec_dumps_match_reference_dumps(gpu_lib, tmp_path, case):
         assert rel_err(got[f], ref) <= TOL64, f
 
 
-def test_exec_dump_schedule_and_float_dumps(gpu_lib, tmp_path):
-    """Dumps at (ts+1) % plotFreq == 0 after warm-up (tLBM.cpp:55-59), FP32
-    storage writes float .b_dat files."""
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f32x", 1e-5), ("f16", 1e-3)])
+def test_exec_dump_schedule_and_float_dumps(gpu_lib, tmp_path, storage, tol):
+    """Dumps at (ts+1) % plotFreq == 0 after warm-up (tLBM.cpp:55-59); FP32,
+    FP32X and FP16 storage write float .b_dat files."""
     nx, ny, nz = 10, 8, 16
     nd = geometry.channel(nx, ny, nz, obstacles=[geometry.sphere(5, 4, 6, 2)])
     p = io_formats.Params(latticeType="D3Q19", numTs=50, tsRepFreq=25, warmupTs=20, plotFreq=10,
                           uLBM=0.05, omega=1.6, nx=nx, ny=ny, nz=nz)
     d = case_dir(tmp_path, p, nd)
-    r = run_exec(d, storage="f32")
+    r = run_exec(d, storage=storage)
     assert r.returncode == 0, r.stderr
     assert "Executing time step: 25" in r.stdout and "Executing time step: 50" in r.stdout
     files = sorted(f for f in os.listdir(d) if f.startswith("ux"))
@@ -66,7 +67,7 @@ def test_exec_dump_schedule_and_float_dumps(gpu_lib, tmp_path):
         fluid = nd == 0
         u_ref = np.stack([ref[c][fluid] for c in ("ux", "uy", "uz")])
         u_got = np.stack([got[c][fluid] for c in ("ux", "uy", "uz")]).astype(np.float64)
-        assert np.linalg.norm(u_got - u_ref) / np.linalg.norm(u_ref) < 1e-5, k
+        assert np.linalg.norm(u_got - u_ref) / np.linalg.norm(u_ref) < tol, k
 
 
 def test_exec_time_average_sums(gpu_lib, tmp_path):

@@ -91,7 +91,7 @@ def test_fp64_random_state_matches_oracle(gpu_lib, geom, q, cs):
 
 @pytest.mark.parametrize("addressing", ["relative", "generic"])
 @pytest.mark.parametrize("q", [15, 19, 27])
-@pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16", "f32x"])
 def test_relative_addressing_lattice_matches_oracle(gpu_lib, q, storage, addressing, monkeypatch):
     """An odd-sized lattice (33x17x40, walls, spheres, inlet/outlet) from a
     random state: FP64 to the FP64 bar, FP32 and FP16 (deviation storage)
@@ -109,7 +109,7 @@ def test_relative_addressing_lattice_matches_oracle(gpu_lib, q, storage, address
     o = Oracle(q, nx, ny, nz, nd, **kw)
     f0 = _random_state(np.random.default_rng(q + 7), o.weights(), nx * ny * nz, amp=0.05)
     o.set_populations(f0)
-    tol = {"f64": TOL64, "f32": 1e-5, "f16": 2e-3}[storage]
+    tol = {"f64": TOL64, "f32": 1e-5, "f16": 2e-3, "f32x": 1e-6}[storage]
     with Lattice(q, nx, ny, nz, nd, storage=storage, **kw) as lat:
         lat.set_populations(f0)
         for nsteps in (1, 5):
@@ -215,7 +215,7 @@ def _velocities(q):
     return np.array([[int(v) for v in ln.split()] for ln in out.strip().splitlines()])
 
 
-@pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16", "f32x"])
 def test_reductions_match_host_sums(gpu_lib, storage):
     nx, ny, nz, nd = GEOMS["channel"]()
     q = 19
@@ -293,7 +293,7 @@ def test_translation_invariance_periodic_box(gpu_lib):
         a = a.reshape(nz, ny, nx, *a.shape[1:])
         return np.roll(a, (shift[2], shift[1], shift[0]), axis=(0, 1, 2)).reshape(nx * ny * nz, *a.shape[3:])
 
-    for storage in ("f64", "f32", "f16"):
+    for storage in ("f64", "f32", "f16", "f32x"):
         outs = []
         for rolled in (False, True):
             with Lattice(q, nx, ny, nz, roll(nd) if rolled else nd, storage=storage, omega=1.7,

@@ -91,4 +91,5 @@ def test_cylinders_runner_short(gpu_lib):
     last = rep["checkpoints"][-1]
     assert last["l2_velocity_f32_vs_f64"] < 1e-5
     assert last["l2_velocity_f16_vs_f64"] < 1e-3
-    assert set(rep["runs"]) == {"f64", "f32", "f16"}
+    assert last["l2_velocity_f32x_vs_f64"] < 1e-6
+    assert set(rep["runs"]) == {"f64", "f32", "f16", "f32x"}

@@ -26,7 +26,7 @@ struct Launch {
         switch (mode) {
         case MODE_STEP:
             if (a.closure) {
-                if constexpr (ST == ST_F64) {
+                if constexpr (ST == ST_F64 || ST == ST_F32X) {
                     return hipErrorInvalidValue;
                 } else {
                     constexpr int CBS = kClosureBlock;

@@ -1,7 +1,8 @@
 // launch.h -- host-side launchers of the kernels, one table per storage.
 // The tables are defined in kernels_f64.hip (FP64, compiled without
 // floating-point contraction so it keeps the reference's rounding) and
-// kernels_f32.hip (FP32 and FP16 storage).
+// kernels_f32.hip (FP32 and FP16 storage) and kernels_f32x.hip (FP32
+// storage with FP64 arithmetic).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,5 +34,6 @@ struct LaunchOps {
 const LaunchOps &ops_f64();
 const LaunchOps &ops_f32();
 const LaunchOps &ops_f16();
+const LaunchOps &ops_f32x();
 
 }  // namespace tlbm

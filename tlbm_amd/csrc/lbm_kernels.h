@@ -41,7 +41,7 @@
 
 namespace tlbm {
 
-enum Storage : int { ST_F64 = 0, ST_F32 = 1, ST_F16 = 2 };
+enum Storage : int { ST_F64 = 0, ST_F32 = 1, ST_F16 = 2, ST_F32X = 3 };
 enum Mode : int { MODE_STEP = 0, MODE_MACROS = 1, MODE_HEAD = 2, MODE_AVG = 3, MODE_STRESS = 4 };
 
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
@@ -126,6 +126,28 @@ template <> struct StorageT<ST_F32> {
         return T(f - double(a.wf[s]));
     }
 };
+// FP32 storage of d = f - w with FP64 arithmetic: the loads widen, the
+// stores round once.
+template <> struct StorageT<ST_F32X> {
+    using st = float;
+    static constexpr int kCB = 13;  // 32 KiB chunks
+    using T = double;
+    static constexpr bool kShift = true;
+    __device__ static inline T in(st r, int, const StepArgs &) { return T(r); }
+    __device__ static inline st out(T v, int, const StepArgs &) { return st(v); }
+    __device__ static inline T load(const st *p, int, const StepArgs &) { return T(*p); }
+    __device__ static inline void store(st *p, int, T v, const StepArgs &) {
+        __builtin_nontemporal_store(st(v), p);
+    }
+    __device__ static inline st step_out(T v, int, const StepArgs &, uint32_t) { return st(v); }
+    __device__ static inline T weight(int s, const StepArgs &a) { return T(a.wf[s]); }
+    __device__ static inline double absolute(T v, int s, const StepArgs &a) {
+        return double(a.wf[s]) + double(v);
+    }
+    __device__ static inline T from_absolute(double f, int s, const StepArgs &a) {
+        return T(st(f - double(a.wf[s])));
+    }
+};
 template <> struct StorageT<ST_F16> {
     using st = _Float16;
     static constexpr int kCB = 14;  // 32 KiB chunks
@@ -196,6 +218,10 @@ template <int Q> struct Tune<ST_F64, Q> {
     static constexpr bool kNTLoad = Q != 27;
 };
 template <int Q> struct Tune<ST_F32, Q> {
+    static constexpr int kBS = 256;
+    static constexpr bool kNTLoad = false;
+};
+template <int Q> struct Tune<ST_F32X, Q> {
     static constexpr int kBS = 256;
     static constexpr bool kNTLoad = false;
 };

@@ -75,12 +75,13 @@ const tlbm::LaunchOps &ops_for(int storage) {
     switch (storage) {
     case TLBM_STORAGE_F64: return tlbm::ops_f64();
     case TLBM_STORAGE_F32: return tlbm::ops_f32();
+    case TLBM_STORAGE_F32X: return tlbm::ops_f32x();
     default: return tlbm::ops_f16();
     }
 }
 
 size_t storage_bytes(int storage) {
-    return storage == TLBM_STORAGE_F64 ? 8 : storage == TLBM_STORAGE_F32 ? 4 : 2;
+    return storage == TLBM_STORAGE_F64 ? 8 : storage == TLBM_STORAGE_F16 ? 2 : 4;
 }
 
 }  // namespace
@@ -154,7 +155,7 @@ int tlbm_gpu_create(const tlbm_params *p, const uint8_t *nd_type, tlbm_gpu **out
     const uint64_t N64 = uint64_t(p->nx) * uint64_t(p->ny) * uint64_t(p->nz);
     if (N64 >= (uint64_t(1) << 31) - 4096)
         return fail(TLBM_EINVAL, "lattice has 2^31 - 4096 nodes or more");
-    if (p->storage < TLBM_STORAGE_F64 || p->storage > TLBM_STORAGE_F16)
+    if (p->storage < TLBM_STORAGE_F64 || p->storage > TLBM_STORAGE_F32X)
         return fail(TLBM_EINVAL, "unknown storage");
     if (p->reserved != 0) return fail(TLBM_EINVAL, "reserved field must be 0");
     if (!(p->omega > 0.0) || !std::isfinite(p->omega)) return fail(TLBM_EINVAL, "omega must be > 0");
@@ -504,7 +505,7 @@ int tlbm_gpu_set_closure(tlbm_gpu *h, const float *params, double feature_scale,
         h->args.closure = 0;
         return TLBM_OK;
     }
-    if (h->p.storage == TLBM_STORAGE_F64)
+    if (h->p.storage == TLBM_STORAGE_F64 || h->p.storage == TLBM_STORAGE_F32X)
         return fail(TLBM_EINVAL, "the learned closure runs with FP32 or FP16 storage");
     if (!(feature_scale > 0.0) || !(output_scale > 0.0) || !std::isfinite(feature_scale) ||
         !std::isfinite(output_scale))

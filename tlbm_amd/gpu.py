@@ -15,8 +15,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtlbm_gpu.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "tlbm_gpu.h")
 
-STORAGE = {"f64": 0, "f32": 1, "f16": 2}
-BYTES_PER_VALUE = {"f64": 8, "f32": 4, "f16": 2}
+STORAGE = {"f64": 0, "f32": 1, "f16": 2, "f32x": 3}
+BYTES_PER_VALUE = {"f64": 8, "f32": 4, "f16": 2, "f32x": 4}
 LATTICE_Q = {"D3Q15": 15, "D3Q19": 19, "D3Q27": 27}
 
 

@@ -27,7 +27,8 @@ int storage_from_env() {
     if (!s || !*s || std::strcmp(s, "f32") == 0) return TLBM_STORAGE_F32;
     if (std::strcmp(s, "f64") == 0) return TLBM_STORAGE_F64;
     if (std::strcmp(s, "f16") == 0) return TLBM_STORAGE_F16;
-    throw std::invalid_argument(std::string("TLBM_STORAGE must be f64, f32 or f16, not ") + s);
+    if (std::strcmp(s, "f32x") == 0) return TLBM_STORAGE_F32X;
+    throw std::invalid_argument(std::string("TLBM_STORAGE must be f64, f32, f16 or f32x, not ") + s);
 }
 
 bool env_flag(const char *name) {

@@ -63,7 +63,7 @@ from . import geometry
 from .closure import uniform_start
 from .gpu import Lattice
 
-STORAGE_BYTES = {"f64": 8, "f32": 4, "f16": 2}
+STORAGE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "f32x": 4}
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_NETWORK = os.path.join(HERE, "closure_d3q19_256x128x128.json")
 
@@ -211,10 +211,10 @@ def run_cylinders(steps: int = 20000, chunk: int = 500,
     """The stream-collide benchmark's lattice over the configuration's full
     length: D3Q19 BGK 128x128x512, periodic x and y, a cylinder along x
     (r 12 at (y, z) = (64, 128)), Zou-He inlet u_z 0.04 and outlet rho 1,
-    tau 0.55, from rest.  FP64, FP32 and FP16 lattices advance side by side;
-    the velocity L2 and mass drift of FP32 and FP16 against FP64, and the
-    cylinder's drag (z momentum exchange) in each, are recorded at the
-    checkpoints."""
+    tau 0.55, from rest.  FP64, FP32, FP16 and FP32X (FP32 storage, FP64
+    arithmetic) lattices advance side by side; the velocity L2 and mass drift
+    of each against FP64, and the cylinder's drag (z momentum exchange) in
+    each, are recorded at the checkpoints."""
     nx, ny, nz = 128, 128, 512
     nd = geometry.channel(nx, ny, nz, walls=(), obstacles=[geometry.cylinder_x(64, 128, 12)])
     N, Q = nd.size, 19
@@ -223,7 +223,7 @@ def run_cylinders(steps: int = 20000, chunk: int = 500,
     out = {"workload": "cylinder_array_d3q19_128x128x512", "steps": steps, "runs": {},
            "checkpoints": []}
     kw = dict(omega=1.0 / 0.55, u_lbm=0.04)
-    lats = {st: Lattice(19, nx, ny, nz, nd, storage=st, **kw) for st in ("f64", "f32", "f16")}
+    lats = {st: Lattice(19, nx, ny, nz, nd, storage=st, **kw) for st in ("f64", "f32", "f16", "f32x")}
     ms = {st: 0.0 for st in lats}
     done = 0
     try:
@@ -237,7 +237,7 @@ def run_cylinders(steps: int = 20000, chunk: int = 500,
                 mass = {st: lat.reduce()["mass"] for st, lat in lats.items()}
                 drag = {st: float(np.sum(lat.forces()["Fz"][body])) for st, lat in lats.items()}
                 cp = {"step": done, "drag": drag}
-                for st in ("f32", "f16"):
+                for st in ("f32", "f16", "f32x"):
                     cp[f"l2_velocity_{st}_vs_f64"] = _l2(m[st], m["f64"], fluid)
                     cp[f"mass_drift_{st}_vs_f64"] = abs(mass[st] - mass["f64"]) / mass["f64"]
                 out["checkpoints"].append(cp)
