@@ -146,15 +146,16 @@ def test_closure_run_follows_oracle(gpu_lib, storage, q):
 
 
 def test_set_closure_rejects_bad_input(gpu_lib):
-    """FP64 storage, non-positive or non-finite scales and non-finite
-    parameters are refused with an error; the lattice keeps running on its
+    """FP64 and FP32X storage, non-positive or non-finite scales and
+    non-finite parameters are refused with an error; the lattice keeps running on its
     own rule and accepts a valid network afterwards."""
     from tlbm_amd.gpu import TlbmError
     nx, ny, nz, nd = _channel()
     params = mlp_init(3)
-    with Lattice(19, nx, ny, nz, nd, storage="f64", omega=1.6, u_lbm=0.05) as a:
-        with pytest.raises(TlbmError, match="FP32 or FP16"):
-            a.set_closure(params, 1.0, 0.01)
+    for storage in ("f64", "f32x"):
+        with Lattice(19, nx, ny, nz, nd, storage=storage, omega=1.6, u_lbm=0.05) as a:
+            with pytest.raises(TlbmError, match="FP32 or FP16"):
+                a.set_closure(params, 1.0, 0.01)
     with Lattice(19, nx, ny, nz, nd, storage="f32", omega=1.6, u_lbm=0.05) as b:
         for fs, os_ in ((0.0, 0.01), (1.0, -0.01), (np.inf, 0.01), (1.0, np.nan)):
             with pytest.raises(TlbmError, match="scales"):
