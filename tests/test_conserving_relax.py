@@ -1,0 +1,70 @@
+"""The identity behind the FP32 collide kernels' conserving relax.
+
+FP32 fluid nodes relax with the rounding residue of the mass and momentum of
+f - feq removed, measured against the values those moments have in exact
+arithmetic with the float weights the kernels use (w_f = float(w)):
+
+    sum_s (f_s - feq_s)      = rho (1.5 k1 |u|^2 - (sum w_f - 1))
+    sum_s e_s (f_s - feq_s)  = k2 rho u
+    k1 = 1 + (sum w_f - 1) - 3 W2,  k2 = 1 - 3 W2,  W2 = sum_s w_f e_x^2
+
+where rho = sum f, u = sum e f / rho and feq_s = w_s rho (1 + 3 e.u +
+4.5 (e.u)^2 - 1.5 |u|^2) (include/LatticeStructure.hpp:92-103).  Here both
+sides are evaluated in exact rational arithmetic on random populations, so a
+dropped term, a wrong sign or a wrong constant fails.  The weights are the
+oracle's tables (equal to the product's, test_oracle.py), rounded to float.
+"""
+from fractions import Fraction as Fr
+
+import numpy as np
+import pytest
+
+import lattice_tables
+
+
+def _float_weights(q):
+    e, _, w, _ = lattice_tables.tables(q)
+    return e, [Fr(float(np.float32(x))) for x in w]
+
+
+@pytest.mark.parametrize("q", [15, 19, 27])
+def test_moments_of_nonequilibrium_match_closed_form(q):
+    e, wf = _float_weights(q)
+    dw = sum(wf) - 1
+    w2 = sum(wf[s] * int(e[s, 0]) ** 2 for s in range(q))
+    # the weights are symmetric: every axis has the same second moment, no cross terms
+    assert all(sum(wf[s] * int(e[s, a]) ** 2 for s in range(q)) == w2 for a in range(3))
+    assert all(sum(wf[s] * int(e[s, a]) * int(e[s, b]) for s in range(q)) == 0
+               for a in range(3) for b in range(3) if a != b)
+    k1, k2 = 1 + dw - 3 * w2, 1 - 3 * w2
+    rng = np.random.default_rng(q)
+    for trial in range(5):
+        f = [wf[s] * (1 + Fr(float(rng.uniform(-0.2, 0.2)))) for s in range(q)]
+        rho = sum(f)
+        u = [sum(f[s] * int(e[s, a]) for s in range(q)) / rho for a in range(3)]
+        usq = sum(c * c for c in u)
+        feq = []
+        for s in range(q):
+            eu = sum(int(e[s, a]) * u[a] for a in range(3))
+            feq.append(wf[s] * rho * (1 + 3 * eu + Fr(9, 2) * eu * eu - Fr(3, 2) * usq))
+        nq = [f[s] - feq[s] for s in range(q)]
+        assert sum(nq) == rho * (Fr(3, 2) * k1 * usq - dw)
+        for a in range(3):
+            assert sum(nq[s] * int(e[s, a]) for s in range(q)) == k2 * rho * u[a]
+
+
+@pytest.mark.parametrize("q", [15, 19, 27])
+def test_correction_removes_a_residue(q):
+    """w_s (m0 + 3 e_s.m) carries mass m0 and momentum m up to the weights'
+    normalisation (sum w_f = 1 + dw, 3 W2 = 1 - k2, both within 1e-7 of exact),
+    so subtracting it from a residue (m0, m) leaves a residue smaller by 1e-7."""
+    e, wf = _float_weights(q)
+    rng = np.random.default_rng(100 + q)
+    m0 = Fr(float(rng.standard_normal()))
+    m = [Fr(float(rng.standard_normal())) for _ in range(3)]
+    corr = [wf[s] * (m0 + 3 * sum(int(e[s, a]) * m[a] for a in range(3))) for s in range(q)]
+    left0 = m0 - sum(corr)
+    assert abs(left0) <= Fr(1, 10 ** 7) * abs(m0)
+    for a in range(3):
+        left = m[a] - sum(corr[s] * int(e[s, a]) for s in range(q))
+        assert abs(left) <= Fr(1, 10 ** 7) * max(abs(x) for x in m)
