@@ -2,17 +2,19 @@
 
 FP32 fluid nodes relax with the rounding residue of the mass and momentum of
 f - feq removed, measured against the values those moments have in exact
-arithmetic with the float weights the kernels use (w_f = float(w)):
+arithmetic with the weights w (the reference's: float values for D3Q15 and
+D3Q19, double values for D3Q27):
 
-    sum_s (f_s - feq_s)      = rho (1.5 k1 |u|^2 - (sum w_f - 1))
+    sum_s (f_s - feq_s)      = rho (1.5 k1 |u|^2 - k0)
     sum_s e_s (f_s - feq_s)  = k2 rho u
-    k1 = 1 + (sum w_f - 1) - 3 W2,  k2 = 1 - 3 W2,  W2 = sum_s w_f e_x^2
+    k0 = sum w - 1,  k1 = 1 + k0 - 3 W2,  k2 = 1 - 3 W2,  W2 = sum_s w e_x^2
 
 where rho = sum f, u = sum e f / rho and feq_s = w_s rho (1 + 3 e.u +
 4.5 (e.u)^2 - 1.5 |u|^2) (include/LatticeStructure.hpp:92-103).  Here both
-sides are evaluated in exact rational arithmetic on random populations, so a
-dropped term, a wrong sign or a wrong constant fails.  The weights are the
-oracle's tables (equal to the product's, test_oracle.py), rounded to float.
+sides are evaluated in exact rational arithmetic on random populations, for
+the weights and for their float roundings (the FP32 kernels' equilibrium
+weights), so a dropped term, a wrong sign or a wrong constant fails.  The
+weights are the oracle's tables (equal to the product's, test_oracle.py).
 """
 from fractions import Fraction as Fr
 
@@ -22,14 +24,27 @@ import pytest
 import lattice_tables
 
 
-def _float_weights(q):
+def _weights(q, rounding):
     e, _, w, _ = lattice_tables.tables(q)
-    return e, [Fr(float(np.float32(x))) for x in w]
+    if rounding == "float":
+        w = [float(np.float32(x)) for x in w]
+    return e, [Fr(float(x)) for x in w]
 
 
+def test_d3q27_float_weights_do_not_sum_to_one():
+    """The reason the targets come from the weights, not their float
+    roundings: the D3Q27 weights sum to 1 to double precision, their float
+    values to 1 + 7.45e-9, a mass source in exact arithmetic."""
+    _, w = _weights(27, "double")
+    _, wf = _weights(27, "float")
+    assert abs(float(sum(w) - 1)) < 1e-15
+    assert abs(float(sum(wf) - 1) - 7.450580596923828e-09) < 1e-15
+
+
+@pytest.mark.parametrize("rounding", ["double", "float"])
 @pytest.mark.parametrize("q", [15, 19, 27])
-def test_moments_of_nonequilibrium_match_closed_form(q):
-    e, wf = _float_weights(q)
+def test_moments_of_nonequilibrium_match_closed_form(q, rounding):
+    e, wf = _weights(q, rounding)
     dw = sum(wf) - 1
     w2 = sum(wf[s] * int(e[s, 0]) ** 2 for s in range(q))
     # the weights are symmetric: every axis has the same second moment, no cross terms
@@ -58,7 +73,7 @@ def test_correction_removes_a_residue(q):
     """w_s (m0 + 3 e_s.m) carries mass m0 and momentum m up to the weights'
     normalisation (sum w_f = 1 + dw, 3 W2 = 1 - k2, both within 1e-7 of exact),
     so subtracting it from a residue (m0, m) leaves a residue smaller by 1e-7."""
-    e, wf = _float_weights(q)
+    e, wf = _weights(q, "float")
     rng = np.random.default_rng(100 + q)
     m0 = Fr(float(rng.standard_normal()))
     m = [Fr(float(rng.standard_normal())) for _ in range(3)]

@@ -44,9 +44,12 @@ def test_porous_runner_small(gpu_lib):
                                checkpoints=(100, 300))
     for st in ("f64", "f32"):
         assert rep["runs"][st]["mlups"] > 0
+    # D3Q27: FP32 follows FP64 this closely because its conserving relax aims
+    # at the moments of the double weights; their float values alone would
+    # add a mass source of about 1.5e-8 per step
     first = rep["checkpoints"][0]
-    assert np.isfinite(first["l2_velocity_vs_f64"]) and first["l2_velocity_vs_f64"] < 1e-3
-    assert first["mass_drift_vs_f64"] < 1e-5
+    assert np.isfinite(first["l2_velocity_vs_f64"]) and first["l2_velocity_vs_f64"] < 1e-5
+    assert first["mass_drift_vs_f64"] < 1e-7
 
 
 @pytest.mark.gpu

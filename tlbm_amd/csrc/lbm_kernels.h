@@ -73,10 +73,11 @@ struct StepArgs {
     float c_in;        // 2*sum_{e_z=-1} w + sum_{e_z=0} w - (1 - u_bc)
     float c_out;       // 2*sum_{e_z=+1} w + sum_{e_z=0} w - rho_bc
     float drho_bc;     // rho_bc - 1
-    // exact-arithmetic moments of f - feq with these float weights (FP32
-    // storage): sum_s (f - feq) = rho (1.5 k1 u^2 - dw), sum_s e (f - feq)
-    // = k2 rho u; k1 = 1 + dw - 3 W2, k2 = 1 - 3 W2, W2 = sum_s w e_x^2
-    float k1, k2;
+    // exact-arithmetic moments of f - feq with the weights w (FP32 storage,
+    // whose float weights may sum differently): sum_s (f - feq) =
+    // rho (1.5 k1 u^2 - k0), sum_s e (f - feq) = k2 rho u; k0 = sum_s w - 1,
+    // k1 = 1 + k0 - 3 W2, k2 = 1 - 3 W2, W2 = sum_s w e_x^2
+    float k0, k1, k2;
     // learned closure (FP32/FP16 storage): tau = tau0 + cl_out * softplus(z),
     // z = network(features * cl_in)
     int closure;
@@ -570,13 +571,16 @@ __device__ inline void node_relax(T (&f)[L::Q], const T (&feq)[L::Q], T om) {
 }
 
 // relax of a fluid node in FP32 storage, conserving to the rounding of
-// f - feq.  In exact arithmetic the mass and momentum of f - feq are the
-// small values the float weights give (their sum is not 1); rounded FP32
-// equilibria miss them by a few ulp of feq, the same amount at every node of
-// a steady flow, and that residue accumulates step after step (D3Q19
-// cylinder array: velocity L2 against FP64 1.6e-5 after 20000 steps).  The
-// residue of mass m0 and momentum m is taken out of f - feq as
-// w_s (m0 + 3 e_s.m), which has exactly that mass and momentum.
+// f - feq.  In exact arithmetic the mass and momentum of f - feq are small
+// values set by the weights (the reference's D3Q15/D3Q19 float weights do
+// not sum to 1); rounded FP32 equilibria miss them by a few ulp of feq, the
+// same amount at every node of a steady flow, and that residue accumulates
+// step after step (D3Q19 cylinder array: velocity L2 against FP64 1.6e-5
+// after 20000 steps).  The values are those of the weights w, not of their
+// float roundings: the D3Q27 weights sum to 1 in double but to 1 + 7.5e-9
+// in float, a mass source FP64 does not have.  The residue of mass m0 and
+// momentum m is taken out of f - feq as w_s (m0 + 3 e_s.m), which carries
+// that mass and momentum.
 template <class L>
 __device__ inline void node_relax_conserving(float (&f)[L::Q], const float (&feq)[L::Q],
                                              const float (&w)[L::Q], float om, float rho,
@@ -591,7 +595,7 @@ __device__ inline void node_relax_conserving(float (&f)[L::Q], const float (&feq
         if (L::ey[s] > 0) my += nq[s]; else if (L::ey[s] < 0) my -= nq[s];
         if (L::ez[s] > 0) mz += nq[s]; else if (L::ez[s] < 0) mz -= nq[s];
     }
-    m0 -= rho * (1.5f * a.k1 * (ux * ux + uy * uy + uz * uz) - a.dw);
+    m0 -= rho * (1.5f * a.k1 * (ux * ux + uy * uy + uz * uz) - a.k0);
     const float rk = rho * a.k2;
     mx = 3.0f * (mx - rk * ux);
     my = 3.0f * (my - rk * uy);

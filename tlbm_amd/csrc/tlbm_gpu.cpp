@@ -54,9 +54,16 @@ void fill_weights(tlbm::StepArgs &a, bool exact) {
     }
     for (int i = 0; i < L::n_zero; ++i) s_zero += double(a.wf[L::zero_list[i]]);
     a.dw = float(sum - 1.0);
-    double w2 = 0;
-    for (int s = 0; s < L::Q; ++s) w2 += double(a.wf[s]) * double(L::ex[s] * L::ex[s]);
-    a.k1 = float(1.0 + (sum - 1.0) - 3.0 * w2);
+    // moments of f - feq in exact arithmetic with the weights w themselves
+    // (D3Q15/D3Q19 reference weights: the float values; D3Q27 and exact
+    // weights: the double values, which sum to 1)
+    double sum_w = 0, w2 = 0;
+    for (int s = 0; s < L::Q; ++s) {
+        sum_w += a.w[s];
+        w2 += a.w[s] * double(L::ex[s] * L::ex[s]);
+    }
+    a.k0 = float(sum_w - 1.0);
+    a.k1 = float(1.0 + (sum_w - 1.0) - 3.0 * w2);
     a.k2 = float(1.0 - 3.0 * w2);
     a.c_in = float(2.0 * s_in + s_zero - (1.0 - a.u_bc));
     a.c_out = float(2.0 * s_out + s_zero - a.rho_bc);
