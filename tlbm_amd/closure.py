@@ -3,7 +3,7 @@
 The reference has one closure, Smagorinsky in the form of
 include/LatticeStructure.hpp:140-153 (nu_t = (sqrt(cs Qn + nu0^2) - nu0)/6,
 with Qn the norm of the non-equilibrium stress).  This module builds the
-learned replacement and judges it against that form:
+learned replacement and compares it with that form:
 
 1. A fine D3Q19 run with Smagorinsky (FP32 storage) past a staggered array
    of spheres, periodic in x and y, Zou-He inlet/outlet in z, with a seeded
