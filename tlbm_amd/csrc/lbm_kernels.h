@@ -208,9 +208,11 @@ constexpr uint32_t kStreamStep = 0x9E3779B9u;
 // * FP64 D3Q19 (128x128x512): 1024-thread workgroups with non-temporal
 //   loads, ahead of 512 threads and of a two-nodes-per-lane 16-byte
 //   variant (20.9 against 19.8 GLUPS, and 13.5 against 12.0 for D3Q27).
-// * FP64 D3Q27 (porous 384^3, 128 VGPRs): 512-thread workgroups with plain
-//   loads, 4.12 ms per step against 4.38 for 1024 threads with
-//   non-temporal loads (4.16 for 512 with them, 4.53 for 256).
+// * FP64 D3Q27 (porous 384^3, 133 VGPRs, 3 waves per SIMD): 512-thread
+//   workgroups with plain loads, 4.12 ms per step against 4.38 for 1024
+//   threads with non-temporal loads (4.16 for 512 with them, 4.53 for 256).
+//   Forced to 4 waves (128 VGPRs and 28 bytes of spills) it took 4.46 ms;
+//   FP32 D3Q27 forced from 4 to 5 waves, 2.057 against 2.043 ms.
 // * FP32: 256-thread workgroups (D3Q19: 203 against 205 us with 512; D3Q27
 //   porous: 2.05 ms, the same as 512 and 1024).
 // * FP16: 512-thread workgroups.
