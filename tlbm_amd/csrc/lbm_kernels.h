@@ -572,7 +572,7 @@ __device__ inline void node_relax(T (&f)[L::Q], const T (&feq)[L::Q], T om) {
     for (int s = 0; s < L::Q; ++s) f[s] = f[s] - om * (f[s] - feq[s]);
 }
 
-// relax of a fluid node in FP32 storage, conserving to the rounding of
+// relax of a fluid node in FP32 (and D3Q27 FP16) storage, conserving to the rounding of
 // f - feq.  In exact arithmetic the mass and momentum of f - feq are small
 // values set by the weights (the reference's D3Q15/D3Q19 float weights do
 // not sum to 1); rounded FP32 equilibria miss them by a few ulp of feq, the
@@ -609,8 +609,16 @@ __device__ inline void node_relax_conserving(float (&f)[L::Q], const float (&feq
     }
 }
 
+// Whether a step kernel relaxes fluid nodes with node_relax_conserving: FP32
+// storage always; FP16 storage for D3Q27, whose float weights carry a mass
+// source of omega * 7.45e-9 per step that the double weights do not (with
+// the D3Q15/D3Q19 float-literal weights FP16 and FP64 share the same mass
+// balance, and FP16's stochastic rounding is no bias).
+template <int ST, int Q>
+constexpr bool kConserve = ST == ST_F32 || (ST == ST_F16 && Q == 27);
+
 // The whole reference update of one node (TLBM_Partition.cpp:742-807).
-// CONSERVE (FP32 storage): fluid nodes relax with node_relax_conserving.
+// CONSERVE: fluid nodes relax with node_relax_conserving.
 template <class L, class T, bool SHIFT, bool COLLIDE, bool CONSERVE = false>
 __device__ inline void node_update(T (&f)[L::Q], int t, T &rho, T &ux, T &uy, T &uz,
                                    const StepArgs &a, const T *__restrict__ uy_head,
@@ -697,7 +705,7 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
     }
     T rho, ux, uy, uz;
     if (MODE == MODE_STEP) {
-        node_update<L, T, S::kShift, true, ST == ST_F32>(f, t, rho, ux, uy, uz, a, uy_head, w, g);
+        node_update<L, T, S::kShift, true, kConserve<ST, Q>>(f, t, rho, ux, uy, uz, a, uy_head, w, g);
         const PushOut<L, ST> po(fout, (blk * BS) >> StorageT<ST>::kCB, g);
         const uint32_t rs = ST == ST_F16 ? round_stream(g, a.step) : 0u;
 #pragma unroll
@@ -807,7 +815,7 @@ k_lbm_closure(const typename StorageT<ST>::st *__restrict__ fin, typename Storag
     const float zz = mlp_wave(feat, &smlp);
     const float om = __builtin_amdgcn_rcpf(a.tau0 + a.cl_out * softplus_f(zz));  // 1 ulp
     if (relaxes && a.dynamics == 1) {
-        if (ST == ST_F32 && t == 0)
+        if (kConserve<ST, Q> && t == 0)
             node_relax_conserving<L>(f, feq, w, om, rho, ux, uy, uz, a);
         else
             node_relax<L, T>(f, feq, T(om));
