@@ -11,7 +11,11 @@ A step is one fused stream-collide launch over the whole lattice with the
 populations resident in HBM.  The headline `value` is MLUPS with FP32
 storage (nx*ny*nz*steps / wall time of the timed steps); FP64 and FP16
 (f - w) storage run the same steps for the velocity L2 error and mass drift
-against FP64, reported under `variants`.  `roofline` prices the FP32 kernel
+against FP64, reported under `variants`, after the warm-up and timed steps.
+With --full-steps 20000 FP32 and FP32X are also measured after the
+configuration's full 20000 steps on one x-row of the lattice (1x128x512: x
+is periodic and geometry and start are uniform in x, so the full lattice
+repeats that row exactly).  `roofline` prices the FP32 kernel
 at 2*Q*4 = 152 algorithmic bytes per lattice update against 8 TB/s, with
 its average duration from HIP events on the stream the kernel runs on.
 `cpu_baseline` times the reference's own MPI executable (built from its
@@ -69,6 +73,27 @@ def run_gpu(storage, nd, steps, warmup):
         "macros": m,
         "mass": red["mass"],
     }
+
+
+def full_length_errors(storages, steps):
+    """Velocity L2 and mass drift against FP64 after `steps` steps on one
+    x-row (1 x NY x NZ) of the benchmark lattice, for the storages whose
+    arithmetic is deterministic per node (FP32, FP32X): every row of the
+    full lattice computes exactly what this row does.  FP16 is left out: its
+    stochastic rounding is independent across x in the full lattice, where
+    it largely averages out of the x-uniform flow, but one row makes it the
+    same at every x (20000 steps: 4.7e-3 on the row, 8.0e-4 on the lattice)."""
+    nd = geometry.channel(1, NY, NZ, walls=(), obstacles=[geometry.cylinder_x(64, 128, 12)])
+    fluid = nd == 0
+    kw = dict(omega=1.0 / TAU, u_lbm=U_IN, rho_lbm=1.0)
+    out = {}
+    for st in ["f64"] + [s for s in storages if s in ("f32", "f32x")]:
+        with Lattice(Q, 1, NY, NZ, nd, storage=st, **kw) as lat:
+            lat.step(steps)
+            out[st] = (lat.macros(), lat.reduce()["mass"])
+    m64, mass64 = out.pop("f64")
+    return {st: {"l2_velocity_vs_f64": l2_velocity(m, m64, fluid),
+                 "mass_drift_vs_f64": abs(mass - mass64) / mass64} for st, (m, mass) in out.items()}
 
 
 def l2_velocity(m, ref, fluid):
@@ -170,6 +195,10 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--full-steps", type=int, default=0,
+                    help="also run one x-row of the lattice this many steps (e.g. 20000, the "
+                         "configuration's length) for the L2 and mass drift against FP64; off by "
+                         "default, since its launches share the timed kernels' names in a trace")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cpu-cores", type=int, default=16)
     ap.add_argument("--cpu-timeout", type=float, default=240.0)
@@ -207,6 +236,11 @@ def main():
             v["l2_velocity_vs_f64"] = l2_velocity(r["macros"], res["f64"]["macros"], fluid)
             v["mass_drift_vs_f64"] = abs(r["mass"] - res["f64"]["mass"]) / res["f64"]["mass"]
         variants[st] = v
+    if args.full_steps > 0 and "f64" in res:
+        print(f"[bench] x-row accuracy run: {args.full_steps} steps", file=sys.stderr, flush=True)
+        for st, e in full_length_errors(storages, args.full_steps).items():
+            variants[st][f"l2_velocity_vs_f64_{args.full_steps}_steps"] = e["l2_velocity_vs_f64"]
+            variants[st][f"mass_drift_vs_f64_{args.full_steps}_steps"] = e["mass_drift_vs_f64"]
     line = {
         "metric": METRIC,
         "value": round(h["mlups"], 1),
