@@ -73,8 +73,8 @@ struct StepArgs {
     float c_in;        // 2*sum_{e_z=-1} w + sum_{e_z=0} w - (1 - u_bc)
     float c_out;       // 2*sum_{e_z=+1} w + sum_{e_z=0} w - rho_bc
     float drho_bc;     // rho_bc - 1
-    // exact-arithmetic moments of f - feq with the weights w (FP32 storage,
-    // whose float weights may sum differently): sum_s (f - feq) =
+    // exact-arithmetic moments of f - feq with the weights w (the conserving
+    // relax; its float weights may sum differently): sum_s (f - feq) =
     // rho (1.5 k1 u^2 - k0), sum_s e (f - feq) = k2 rho u; k0 = sum_s w - 1,
     // k1 = 1 + k0 - 3 W2, k2 = 1 - 3 W2, W2 = sum_s w e_x^2
     float k0, k1, k2;
@@ -572,8 +572,8 @@ __device__ inline void node_relax(T (&f)[L::Q], const T (&feq)[L::Q], T om) {
     for (int s = 0; s < L::Q; ++s) f[s] = f[s] - om * (f[s] - feq[s]);
 }
 
-// relax of a fluid node in FP32 (and D3Q27 FP16) storage, conserving to the rounding of
-// f - feq.  In exact arithmetic the mass and momentum of f - feq are small
+// relax of a fluid node in FP32 (and D3Q27 FP16) storage, conserving to
+// the rounding of f - feq.  In exact arithmetic the mass and momentum of f - feq are small
 // values set by the weights (the reference's D3Q15/D3Q19 float weights do
 // not sum to 1); rounded FP32 equilibria miss them by a few ulp of feq, the
 // same amount at every node of a steady flow, and that residue accumulates
