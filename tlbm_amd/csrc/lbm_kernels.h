@@ -217,7 +217,8 @@ constexpr uint32_t kStreamStep = 0x9E3779B9u;
 //   porous: 2.05 ms, the same as 512 and 1024).
 // * FP16: 512-thread workgroups.
 // FP32 and FP16 use plain loads: FP16 109.7 against 111.5 us per step with
-// non-temporal loads; FP32 D3Q27 2.05 against 2.17 ms.  Every kernel stores
+// non-temporal loads; FP32 D3Q19 202.9 against 205.0, D3Q27 2.05 against
+// 2.17 ms.  Every kernel stores
 // non-temporally.
 template <int ST, int Q> struct Tune;
 template <int Q> struct Tune<ST_F64, Q> {
