@@ -261,13 +261,16 @@ def _log(msg: str) -> None:
     print(f"[closure] {msg}", flush=True)
 
 
-def fine_run(case: ClosureCase, storage: str = "f32", keep_raw: int = 0):
+def fine_run(case: ClosureCase, storage: str = "f32", keep_raw: int = 0, inlet_scale=None):
     """Runs the fine lattice and returns the filtered snapshots' targets,
     the filtered velocity and density of every snapshot, and the coarse
-    mask; the last `keep_raw` snapshots' filtered planes are kept whole."""
+    mask; the last `keep_raw` snapshots' filtered planes are kept whole.
+    `inlet_scale` (nx*ny factors) multiplies the inlet profile."""
     nd = fine_geometry(case)
     keep = training_mask(nd, case)
     prof, _ = inlet_profiles(case)
+    if inlet_scale is not None:
+        prof = prof * inlet_scale
     cx, cy, cz = case.coarse
     data = {"rho": [], "S": [], "dtau": [], "weight": [], "u": [], "rho_full": [], "raw": []}
     t0 = time.time()
@@ -293,6 +296,41 @@ def fine_run(case: ClosureCase, storage: str = "f32", keep_raw: int = 0):
     _log(f"fine run {case.nx}x{case.ny}x{case.nz}: {case.spinup + case.snapshots * case.every} "
          f"steps, {case.snapshots} snapshots, {time.time() - t0:.1f} s")
     return nd, keep, data
+
+
+def precision_check(case: ClosureCase, perturbation: float = 1e-7) -> dict:
+    """The fine run in FP32 storage (the pipeline's) against the same run in
+    FP64, next to a second FP64 run whose inlet profile is perturbed by
+    `perturbation` (relative, seeded), which measures how far the unsteady
+    flow itself carries a difference that small.  Per snapshot: the
+    relative L2 of the filtered velocity over the coarse lattice, and the
+    drift of the filtered total density (a 2x2x2 mean over every node, so
+    its sum is the lattice mass over 8).  Over all snapshots: the relative
+    L2 of the time-mean filtered velocity."""
+    _, _, d64 = fine_run(case, storage="f64")
+    _, _, d32 = fine_run(case, storage="f32")
+    rng = np.random.default_rng(7)
+    scale = 1.0 + perturbation * rng.uniform(-1.0, 1.0, case.nx * case.ny)
+    _, _, dp = fine_run(case, storage="f64", inlet_scale=scale)
+
+    def l2(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    rows = []
+    for k in range(case.snapshots):
+        u64 = d64["u"][k].astype(np.float64)
+        m64 = float(d64["rho_full"][k].sum(dtype=np.float64))
+        m32 = float(d32["rho_full"][k].sum(dtype=np.float64))
+        rows.append({"snapshot": k, "step": case.spinup + (k + 1) * case.every,
+                     "l2_velocity_vs_f64": l2(d32["u"][k].astype(np.float64), u64),
+                     "l2_velocity_perturbed_f64": l2(dp["u"][k].astype(np.float64), u64),
+                     "mass_drift_vs_f64": abs(m32 - m64) / m64})
+    mean = {k: np.mean([u.astype(np.float64) for u in d[k2]], axis=0)
+            for k, d, k2 in (("f64", d64, "u"), ("f32", d32, "u"), ("p", dp, "u"))}
+    return {"lattice": [case.nx, case.ny, case.nz], "perturbation": perturbation,
+            "mean_velocity_l2_vs_f64": l2(mean["f32"], mean["f64"]),
+            "mean_velocity_l2_perturbed_f64": l2(mean["p"], mean["f64"]),
+            "snapshots": rows}
 
 
 def features_at(rho, S, tau, scale):
@@ -631,10 +669,20 @@ def main(argv=None) -> int:
     ap.add_argument("--dump-snapshots", type=int, default=8)
     ap.add_argument("--save", default=None,
                     help="write the trained network (params, feature and output scale) as JSON")
+    ap.add_argument("--precision", action="store_true",
+                    help="only compare the fine run in FP32 with the same run in FP64")
     ap.add_argument("--cost-only", default=None, metavar="NETWORK",
                     help="only time the saved network's collide against Smagorinsky "
                          "(D3Q19 128x128x512, FP32 and FP16)")
     a = ap.parse_args(argv)
+    if a.precision:
+        rep = precision_check(ClosureCase())
+        if a.out:
+            with open(a.out, "w") as fh:
+                json.dump(rep, fh)
+        print(json.dumps({k: v for k, v in rep.items() if k != "snapshots"}, indent=1))
+        print(json.dumps(rep["snapshots"][:: max(1, len(rep["snapshots"]) // 10)], indent=1))
+        return 0
     if a.cost_only:
         net = load_network(a.cost_only)
         cost = {f"128x128x512_{st}": step_cost((128, 128, 512), net, storage=st)
