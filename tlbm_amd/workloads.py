@@ -207,23 +207,35 @@ def run_porous(n: int = 384, steps: int = 5000, chunk: int = 500, seed: int = 20
 
 
 def run_cylinders(steps: int = 20000, chunk: int = 500,
-                  checkpoints=(1100, 2000, 5000, 10000, 20000)):
+                  checkpoints=(1100, 2000, 5000, 10000, 20000), q: int = 19, nx: int = 128,
+                  perturb: float = 0.0):
     """The stream-collide benchmark's lattice over the configuration's full
-    length: D3Q19 BGK 128x128x512, periodic x and y, a cylinder along x
+    length: D3Q19 (or `q`) BGK 128x128x512, periodic x and y, a cylinder along x
     (r 12 at (y, z) = (64, 128)), Zou-He inlet u_z 0.04 and outlet rho 1,
     tau 0.55, from rest.  FP64, FP32, FP16 and FP32X (FP32 storage, FP64
     arithmetic) lattices advance side by side; the velocity L2 and mass drift
     of each against FP64, and the cylinder's drag (z momentum exchange) in
-    each, are recorded at the checkpoints."""
-    nx, ny, nz = 128, 128, 512
+    each, are recorded at the checkpoints.  nx = 1 runs one x-row, which the
+    full lattice repeats exactly for FP64 and FP32 (x is periodic, geometry
+    and start uniform in x).  perturb > 0 adds an FP64 lattice whose inlet
+    profile is multiplied by 1 + perturb U[-1, 1] (seeded): how far the flow
+    itself carries a difference that small."""
+    ny, nz = 128, 512
     nd = geometry.channel(nx, ny, nz, walls=(), obstacles=[geometry.cylinder_x(64, 128, 12)])
-    N, Q = nd.size, 19
+    N, Q = nd.size, q
     fluid = nd != geometry.SOLID
     body = nd == geometry.SOLID
-    out = {"workload": "cylinder_array_d3q19_128x128x512", "steps": steps, "runs": {},
+    out = {"workload": f"cylinder_array_d3q{q}_{nx}x128x512", "steps": steps, "runs": {},
            "checkpoints": []}
     kw = dict(omega=1.0 / 0.55, u_lbm=0.04)
-    lats = {st: Lattice(19, nx, ny, nz, nd, storage=st, **kw) for st in ("f64", "f32", "f16", "f32x")}
+    lats = {st: Lattice(q, nx, ny, nz, nd, storage=st, **kw) for st in ("f64", "f32", "f16", "f32x")}
+    compared = ["f32", "f16", "f32x"]
+    if perturb > 0:
+        lats["f64p"] = Lattice(q, nx, ny, nz, nd, storage="f64", **kw)
+        rng = np.random.default_rng(5)
+        lats["f64p"].set_inlet_profile(0.04 * (1.0 + perturb * rng.uniform(-1.0, 1.0, nx * ny)))
+        compared.append("f64p")
+        out["perturbation"] = perturb
     ms = {st: 0.0 for st in lats}
     done = 0
     try:
@@ -237,14 +249,14 @@ def run_cylinders(steps: int = 20000, chunk: int = 500,
                 mass = {st: lat.reduce()["mass"] for st, lat in lats.items()}
                 drag = {st: float(np.sum(lat.forces()["Fz"][body])) for st, lat in lats.items()}
                 cp = {"step": done, "drag": drag}
-                for st in ("f32", "f16", "f32x"):
+                for st in compared:
                     cp[f"l2_velocity_{st}_vs_f64"] = _l2(m[st], m["f64"], fluid)
                     cp[f"mass_drift_{st}_vs_f64"] = abs(mass[st] - mass["f64"]) / mass["f64"]
                 out["checkpoints"].append(cp)
                 _log(f"cylinders step {done}: {cp}")
                 del m
         for st, lat in lats.items():
-            out["runs"][st] = _rate(N, Q, st, steps, ms[st])
+            out["runs"][st] = _rate(N, Q, "f64" if st == "f64p" else st, steps, ms[st])
     finally:
         for lat in lats.values():
             lat.close()
@@ -399,6 +411,10 @@ def main(argv=None) -> int:
     s.add_argument("--no-downsampled", action="store_true")
     c = sub.add_parser("cylinders", help="the benchmark lattice over its full 20000 steps")
     c.add_argument("--steps", type=int, default=20000)
+    c.add_argument("--q", type=int, default=19, choices=(15, 19, 27), help="lattice (D3Qq)")
+    c.add_argument("--nx", type=int, default=128, help="1: one x-row of the lattice")
+    c.add_argument("--perturb", type=float, default=0.0,
+                   help="also run FP64 with the inlet perturbed by this much (relative)")
     b = sub.add_parser("slab", help="one GPU's share of the spheroid lattice split into z-slabs")
     b.add_argument("--gpus", type=int, default=8)
     b.add_argument("--steps", type=int, default=1000)
@@ -410,7 +426,7 @@ def main(argv=None) -> int:
     if a.which == "porous":
         rep = run_porous(a.n, a.steps, seed=a.seed, ramp=a.ramp)
     elif a.which == "cylinders":
-        rep = run_cylinders(a.steps)
+        rep = run_cylinders(a.steps, q=a.q, nx=a.nx, perturb=a.perturb)
     elif a.which == "slab":
         rep = run_slab(a.gpus, a.steps, scale=a.scale, network=a.network)
     else:
