@@ -96,3 +96,18 @@ def test_cylinders_runner_short(gpu_lib):
     assert last["l2_velocity_f16_vs_f64"] < 1e-3
     assert last["l2_velocity_f32x_vs_f64"] < 1e-6
     assert set(rep["runs"]) == {"f64", "f32", "f16", "f32x"}
+
+
+@pytest.mark.gpu
+def test_cylinders_runner_row_with_perturbation(gpu_lib):
+    """One x-row of the D3Q15 cylinder array with the perturbed FP64 run:
+    after 20 steps the 1e-7-perturbed FP64 run and FP32 stay within 1e-5 of
+    FP64 (the perturbation is not zero), FP16 within 1e-2."""
+    rep = workloads.run_cylinders(steps=20, chunk=10, checkpoints=(10, 20), q=15, nx=1,
+                                  perturb=1e-7)
+    assert rep["workload"] == "cylinder_array_d3q15_1x128x512"
+    assert set(rep["runs"]) == {"f64", "f32", "f16", "f32x", "f64p"}
+    last = rep["checkpoints"][-1]
+    assert 0 < last["l2_velocity_f64p_vs_f64"] < 1e-5
+    assert last["l2_velocity_f32_vs_f64"] < 1e-5
+    assert last["l2_velocity_f16_vs_f64"] < 1e-2
