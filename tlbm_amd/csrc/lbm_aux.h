@@ -59,7 +59,7 @@ k_init_macros(typename StorageT<ST>::st *__restrict__ fout, const double *__rest
                      rho = mac[3 * size_t(a.N) + src];
         T v;
         if constexpr (S::kShift) {
-            const double w = double(a.wf[s]);
+            const double w = S::shift(s, a);
             const double usq = ux * ux + uy * uy + uz * uz;
             const double cu = (s == 0) ? 0.0 : 3.0 * edot<double>(L::ex[s], L::ey[s], L::ez[s], ux, uy, uz);
             v = T(w * ((rho - 1.0) + rho * (cu + 0.5 * (cu * cu) - 1.5 * usq)));
@@ -83,7 +83,7 @@ k_init_rest(typename StorageT<ST>::st *__restrict__ fout, StepArgs a) {
     for (int s = 0; s < L::Q; ++s) {
         T v;
         if constexpr (S::kShift)
-            v = T(double(a.wf[s]) * a.rho_bc - double(a.wf[s]));
+            v = T(S::shift(s, a) * a.rho_bc - S::shift(s, a));
         else
             v = S::weight(s, a) * T(a.rho_bc);
         S::store(fout + fidx<L::Q>(s, h, a), s, v, a);

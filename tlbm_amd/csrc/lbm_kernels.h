@@ -73,6 +73,9 @@ struct StepArgs {
     float c_in;        // 2*sum_{e_z=-1} w + sum_{e_z=0} w - (1 - u_bc)
     float c_out;       // 2*sum_{e_z=+1} w + sum_{e_z=0} w - rho_bc
     float drho_bc;     // rho_bc - 1
+    // the same deviation constants from the weights w in double (FP32X,
+    // whose arithmetic and shift are FP64)
+    double dw_d, c_in_d, c_out_d;
     // exact-arithmetic moments of f - feq with the weights w (the conserving
     // relax; its float weights may sum differently): sum_s (f - feq) =
     // rho (1.5 k1 u^2 - k0), sum_s e (f - feq) = k2 rho u; k0 = sum_s w - 1,
@@ -116,6 +119,8 @@ template <> struct StorageT<ST_F32> {
     static constexpr int kCB = 13;  // 32 KiB chunks
     using T = float;
     static constexpr bool kShift = true;
+    // the weight the stored deviation is taken from
+    __device__ static inline double shift(int s, const StepArgs &a) { return double(a.wf[s]); }
     __device__ static inline T in(st r, int, const StepArgs &) { return r; }
     __device__ static inline st out(T v, int, const StepArgs &) { return v; }
     __device__ static inline T load(const st *p, int, const StepArgs &) { return *p; }
@@ -131,13 +136,15 @@ template <> struct StorageT<ST_F32> {
         return T(f - double(a.wf[s]));
     }
 };
-// FP32 storage of d = f - w with FP64 arithmetic: the loads widen, the
-// stores round once.
+// FP32 storage of d = f - w with FP64 arithmetic and the FP64 weights w
+// (for D3Q27 their float roundings sum to 1 + 7.45e-9, a mass source): the
+// loads widen, the stores round once.
 template <> struct StorageT<ST_F32X> {
     using st = float;
     static constexpr int kCB = 13;  // 32 KiB chunks
     using T = double;
     static constexpr bool kShift = true;
+    __device__ static inline double shift(int s, const StepArgs &a) { return a.w[s]; }
     __device__ static inline T in(st r, int, const StepArgs &) { return T(r); }
     __device__ static inline st out(T v, int, const StepArgs &) { return st(v); }
     __device__ static inline T load(const st *p, int, const StepArgs &) { return T(*p); }
@@ -145,12 +152,12 @@ template <> struct StorageT<ST_F32X> {
         __builtin_nontemporal_store(st(v), p);
     }
     __device__ static inline st step_out(T v, int, const StepArgs &, uint32_t) { return st(v); }
-    __device__ static inline T weight(int s, const StepArgs &a) { return T(a.wf[s]); }
+    __device__ static inline T weight(int s, const StepArgs &a) { return a.w[s]; }
     __device__ static inline double absolute(T v, int s, const StepArgs &a) {
-        return double(a.wf[s]) + double(v);
+        return a.w[s] + double(v);
     }
     __device__ static inline T from_absolute(double f, int s, const StepArgs &a) {
-        return T(st(f - double(a.wf[s])));
+        return T(st(f - a.w[s]));
     }
 };
 template <> struct StorageT<ST_F16> {
@@ -158,6 +165,7 @@ template <> struct StorageT<ST_F16> {
     static constexpr int kCB = 14;  // 32 KiB chunks
     using T = float;
     static constexpr bool kShift = true;
+    __device__ static inline double shift(int s, const StepArgs &a) { return double(a.wf[s]); }
     // one mixed-precision FMA (f16 operand, f32 scale)
     __device__ static inline T in(st r, int s, const StepArgs &a) {
         return __builtin_fmaf(float(r), a.h_unscale[s], 0.0f);
@@ -412,7 +420,7 @@ __device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho,
             if (L::ez[s] > 0) mz += f[s]; else if (L::ez[s] < 0) mz -= f[s];
         }
         if constexpr (SHIFT) {
-            dlt = r + T(a.dw);
+            dlt = r + (sizeof(T) == 8 ? T(a.dw_d) : T(a.dw));
             rho = T(1) + dlt;
             // a correctly rounded 1/rho: a 1-ulp reciprocal biases u and
             // doubles the FP32 velocity error against FP64 over 1000 steps
@@ -451,13 +459,14 @@ __device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho,
 #pragma unroll
         for (int i = 0; i < L::n_zero; ++i) sz += f[L::zero_list[i]];
         ux = T(0); uy = T(0);
-        T inv_1mu = T(a.inv_1mu), c_in = T(a.c_in);
+        const T c_in0 = sizeof(T) == 8 ? T(a.c_in_d) : T(a.c_in);
+        T inv_1mu = T(a.inv_1mu), c_in = c_in0;
         if (a.inlet_uz) {
             // a prescribed inlet profile: 1/(1-u) and the deviation constant per node
             const T u = T(a.inlet_uz[g % a.nxny]);
             uz = u;
             inv_1mu = T(1) / (T(1) - u);
-            c_in = T(a.c_in) + (u - T(a.u_bc));
+            c_in = c_in0 + (u - T(a.u_bc));
         } else {
             uz = T(a.u_bc);
         }
@@ -476,8 +485,8 @@ __device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho,
         for (int i = 0; i < L::n_zero; ++i) sz += f[L::zero_list[i]];
         rho = T(a.rho_bc);
         if constexpr (SHIFT) {
-            dlt = T(a.drho_bc);
-            uz = T(a.inv_rho_bc) * (T(a.c_out) + (T(2) * s_out + sz));
+            dlt = sizeof(T) == 8 ? T(a.rho_bc - 1.0) : T(a.drho_bc);
+            uz = T(a.inv_rho_bc) * ((sizeof(T) == 8 ? T(a.c_out_d) : T(a.c_out)) + (T(2) * s_out + sz));
         } else {
             uz = T(-1) + T(a.inv_rho_bc) * (T(2) * s_out + sz);
         }

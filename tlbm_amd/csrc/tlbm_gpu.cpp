@@ -66,6 +66,18 @@ void fill_weights(tlbm::StepArgs &a, bool exact) {
     a.k1 = float(1.0 + (sum_w - 1.0) - 3.0 * w2);
     a.k2 = float(1.0 - 3.0 * w2);
     a.c_in = float(2.0 * s_in + s_zero - (1.0 - a.u_bc));
+    {
+        double sum_d = 0, in_d = 0, out_d = 0, zero_d = 0;
+        for (int s = 0; s < L::Q; ++s) sum_d += a.w[s];
+        for (int i = 0; i < L::n_unk; ++i) {
+            in_d += a.w[L::in_list[i]];
+            out_d += a.w[L::out_list[i]];
+        }
+        for (int i = 0; i < L::n_zero; ++i) zero_d += a.w[L::zero_list[i]];
+        a.dw_d = sum_d - 1.0;
+        a.c_in_d = 2.0 * in_d + zero_d - (1.0 - a.u_bc);
+        a.c_out_d = 2.0 * out_d + zero_d - a.rho_bc;
+    }
     a.c_out = float(2.0 * s_out + s_zero - a.rho_bc);
     a.drho_bc = float(a.rho_bc - 1.0);
 }
