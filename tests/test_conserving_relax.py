@@ -83,3 +83,43 @@ def test_correction_removes_a_residue(q):
     for a in range(3):
         left = m[a] - sum(corr[s] * int(e[s, a]) for s in range(q))
         assert abs(left) <= Fr(1, 10 ** 7) * max(abs(x) for x in m)
+
+
+def test_fp16_d3q27_equilibrium_correction():
+    """FP16 D3Q27 builds its equilibrium from the float weights w_f and
+    corrects it to the balance of the double weights w: 3 e.u becomes
+    t3 e.u with t3 = 3 W2 / W2_f, and the rest population gains
+    rho (1.5 dk1 u^2 - dk0) with dk0 = k0_f - k0, dk1 = k1_f - k1.  In exact
+    arithmetic the momentum of f - feq is then k2 rho u exactly, and its
+    mass rho (1.5 k1 u^2 - k0) up to the t3^2 change of the quadratic term
+    (4.5 rho W2_f u^2 (c^2 - 1), c = W2 / W2_f: about 1e-11 here, against
+    the 7.45e-9 that the correction removes)."""
+    e, w = _weights(27, "double")
+    _, wf = _weights(27, "float")
+    q = 27
+    k0, k0f = sum(w) - 1, sum(wf) - 1
+    W2 = sum(w[s] * int(e[s, 0]) ** 2 for s in range(q))
+    W2f = sum(wf[s] * int(e[s, 0]) ** 2 for s in range(q))
+    k1, k1f, k2 = 1 + k0 - 3 * W2, 1 + k0f - 3 * W2f, 1 - 3 * W2
+    c = W2 / W2f
+    dk0, dk1 = k0f - k0, k1f - k1
+    assert abs(float(dk0) - 7.450580596923828e-09) < 1e-15
+    rng = np.random.default_rng(27)
+    for trial in range(3):
+        f = [wf[s] * (1 + Fr(float(rng.uniform(-0.2, 0.2)))) for s in range(q)]
+        rho = sum(f)
+        u = [sum(f[s] * int(e[s, a]) for s in range(q)) / rho for a in range(3)]
+        usq = sum(x * x for x in u)
+        feq = []
+        for s in range(q):
+            eu = c * sum(int(e[s, a]) * u[a] for a in range(3))
+            feq.append(wf[s] * rho * (1 + 3 * eu + Fr(9, 2) * eu * eu - Fr(3, 2) * usq))
+        feq[0] += rho * (Fr(3, 2) * dk1 * usq - dk0)
+        nq = [f[s] - feq[s] for s in range(q)]
+        for a in range(3):
+            assert sum(nq[s] * int(e[s, a]) for s in range(q)) == k2 * rho * u[a]
+        target = rho * (Fr(3, 2) * k1 * usq - k0)
+        bound = Fr(9, 2) * rho * W2f * usq * abs(c * c - 1) * Fr(101, 100)
+        assert abs(sum(nq) - target) <= bound
+        # without the correction the mass balance misses by ~rho dk0
+        assert abs(float(dk0)) > 100 * float(bound)

@@ -384,9 +384,9 @@ def test_conserves_mass_with_weights_that_sum_to_one(gpu_lib, q, storage):
     step conserves mass in a periodic box with a solid sphere.  The FP32 and
     FP16 kernels compute the equilibrium with those weights rounded to float,
     whose sum differs from 1 by ~1e-8, a mass source of omega times that per
-    step in exact arithmetic; the conserving relax (FP32, and FP16 for
-    D3Q27) aims at the moments of the weights themselves, so the mass is
-    kept to rounding as well.  2000 steps from a random state at omega 1.7:
+    step in exact arithmetic.  The FP32 conserving relax aims at the moments
+    of the weights themselves, and the FP16 D3Q27 equilibrium is corrected to
+    their balance, so the mass is kept to rounding as well.  2000 steps from a random state at omega 1.7:
     without that aim the drift would be 2.5-5e-5 (FP16's unbiased stochastic
     rounding adds a random walk of ~1e-8)."""
     nx, ny, nz = 24, 20, 16

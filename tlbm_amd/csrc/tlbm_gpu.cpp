@@ -36,7 +36,7 @@ int fail(int code, const std::string &msg) {
     } while (0)
 
 template <class L>
-void fill_weights(tlbm::StepArgs &a, bool exact) {
+void fill_weights(tlbm::StepArgs &a, bool exact, int storage) {
     for (int s = 0; s < L::Q; ++s) {
         a.w[s] = exact ? L::w_exact[s] : L::w_ref[s];
         a.wf[s] = float(a.w[s]);
@@ -65,6 +65,18 @@ void fill_weights(tlbm::StepArgs &a, bool exact) {
     a.k0 = float(sum_w - 1.0);
     a.k1 = float(1.0 + (sum_w - 1.0) - 3.0 * w2);
     a.k2 = float(1.0 - 3.0 * w2);
+    // FP16 D3Q27: equilibrium corrections from the float weights' balance to
+    // that of w (zero where the two coincide)
+    a.t3 = 3.0f;
+    a.dk0 = a.dk1 = 0.0f;
+    if (storage == TLBM_STORAGE_F16 && L::Q == 27) {
+        double w2f = 0;
+        for (int s = 0; s < L::Q; ++s) w2f += double(a.wf[s]) * double(L::ex[s] * L::ex[s]);
+        const double k1f = 1.0 + (sum - 1.0) - 3.0 * w2f;
+        a.t3 = float(3.0 * w2 / w2f);
+        a.dk0 = float((sum - 1.0) - (sum_w - 1.0));
+        a.dk1 = float(k1f - (1.0 + (sum_w - 1.0) - 3.0 * w2));
+    }
     a.c_in = float(2.0 * s_in + s_zero - (1.0 - a.u_bc));
     {
         double sum_d = 0, in_d = 0, out_d = 0, zero_d = 0;
@@ -243,9 +255,9 @@ int tlbm_gpu_create(const tlbm_params *p, const uint8_t *nd_type, tlbm_gpu **out
         if (gen && gen[0] == '1') a.rel = 0;
     }
     switch (Q) {
-    case 15: fill_weights<tlbm::D3Q15>(a, p->exact_weights != 0); break;
-    case 19: fill_weights<tlbm::D3Q19>(a, p->exact_weights != 0); break;
-    default: fill_weights<tlbm::D3Q27>(a, p->exact_weights != 0); break;
+    case 15: fill_weights<tlbm::D3Q15>(a, p->exact_weights != 0, p->storage); break;
+    case 19: fill_weights<tlbm::D3Q19>(a, p->exact_weights != 0, p->storage); break;
+    default: fill_weights<tlbm::D3Q27>(a, p->exact_weights != 0, p->storage); break;
     }
     auto cleanup_fail = [&](int code) {
         tlbm_gpu_destroy(h);
