@@ -30,10 +30,11 @@ extern "C" {
 /* Population storage in HBM.  F64 stores f_s and computes in FP64 with the
  * reference's operation order.  F32 stores d_s = f_s - w_s and F16 the
  * scaled deviation (f_s - w_s)/w_s (each step's store rounded
- * stochastically); both compute in FP32 on d_s, and F32 (F16 for D3Q27)
- * relaxes fluid nodes keeping the exact-arithmetic mass and momentum of
- * f - feq to rounding.  F32X stores d_s as F32 does and computes in FP64
- * (no reference counterpart). */
+ * stochastically); both compute in FP32 on d_s.  F32 relaxes fluid nodes
+ * keeping the exact-arithmetic mass and momentum of f - feq to rounding;
+ * F16 D3Q27 corrects its float-weight equilibrium to the balance of the
+ * double weights.  F32X stores d_s as F32 does and computes in FP64 with
+ * the FP64 weights (no reference counterpart). */
 #define TLBM_STORAGE_F64 0
 #define TLBM_STORAGE_F32 1
 #define TLBM_STORAGE_F16 2
