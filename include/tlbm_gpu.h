@@ -32,8 +32,8 @@ extern "C" {
  * scaled deviation (f_s - w_s)/w_s (each step's store rounded
  * stochastically); both compute in FP32 on d_s.  F32 relaxes fluid nodes
  * keeping the exact-arithmetic mass and momentum of f - feq to rounding;
- * F16 D3Q27 corrects its float-weight equilibrium to the balance of the
- * double weights.  F32X stores d_s as F32 does and computes in FP64 with
+ * F16 corrects its float-weight equilibrium to the balance of the weights
+ * where the two differ (D3Q27, exact weights).  F32X stores d_s as F32 does and computes in FP64 with
  * the FP64 weights (no reference counterpart). */
 #define TLBM_STORAGE_F64 0
 #define TLBM_STORAGE_F32 1

@@ -378,7 +378,8 @@ def test_fp32_long_run_stays_within_target(gpu_lib):
             assert drift <= 1e-7, drift
 
 
-@pytest.mark.parametrize("q,storage", [(15, "f32"), (19, "f32"), (27, "f32"), (27, "f16"),
+@pytest.mark.parametrize("q,storage", [(15, "f32"), (19, "f32"), (27, "f32"),
+                                       (15, "f16"), (19, "f16"), (27, "f16"),
                                        (15, "f32x"), (19, "f32x"), (27, "f32x")])
 def test_conserves_mass_with_weights_that_sum_to_one(gpu_lib, q, storage):
     """With exact weights (for D3Q27 the reference's own weights) the FP64
@@ -386,8 +387,9 @@ def test_conserves_mass_with_weights_that_sum_to_one(gpu_lib, q, storage):
     FP16 kernels compute the equilibrium with those weights rounded to float,
     whose sum differs from 1 by ~1e-8, a mass source of omega times that per
     step in exact arithmetic.  The FP32 conserving relax aims at the moments
-    of the weights themselves, and the FP16 D3Q27 equilibrium is corrected to
-    their balance, and FP32X computes with the double weights, so the mass
+    of the weights themselves, and the FP16 equilibrium is corrected to
+    their balance (D3Q15 and D3Q19 exact weights too), and FP32X computes
+    with the double weights, so the mass
     is kept to rounding as well.  2000 steps from a random state at omega 1.7:
     without that aim the drift would be 2.5-5e-5 (FP16's unbiased stochastic
     rounding adds a random walk of ~1e-8)."""

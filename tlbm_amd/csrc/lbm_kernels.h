@@ -76,10 +76,11 @@ struct StepArgs {
     // the same deviation constants from the weights w in double (FP32X,
     // whose arithmetic and shift are FP64)
     double dw_d, c_in_d, c_out_d;
-    // D3Q27 FP16: equilibrium corrections that give the float weights the
-    // exact-arithmetic mass and momentum balance of the double weights:
-    // 3 e.u is scaled to t3 e.u (t3 = 3 W2 / W2_f) and the rest population
-    // gains rho (1.5 dk1 u^2 - dk0); t3 = 3 and dk0 = dk1 = 0 elsewhere
+    // FP16: equilibrium corrections that give the float weights the
+    // exact-arithmetic mass and momentum balance of the weights w (D3Q27 and
+    // exact weights): 3 e.u is scaled to t3 e.u (t3 = 3 W2 / W2_f) and the
+    // rest population gains rho (1.5 dk1 u^2 - dk0); t3 = 3 and
+    // dk0 = dk1 = 0 where the two coincide and in the other storages
     float t3, dk0, dk1;
     // exact-arithmetic moments of f - feq with the weights w (the conserving
     // relax; its float weights may sum differently): sum_s (f - feq) =
@@ -519,14 +520,14 @@ __device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho,
             // A = delta - 3/2 rho u^2 (the same polynomial, fewer operations)
             const T A = dlt - T(1.5) * rho * usq;
             const T h = T(0.5) * __builtin_amdgcn_rcpf(rho);
-            const T t3 = L::Q == 27 ? T(a.t3) : T(3);
+            const T t3 = T(a.t3);
             const T Ux = t3 * rho * ux, Uy = t3 * rho * uy, Uz = t3 * rho * uz;
 #pragma unroll
             for (int s = 0; s < Q; ++s) {
                 const T t = (s == 0) ? T(0) : edot<T>(L::ex[s], L::ey[s], L::ez[s], Ux, Uy, Uz);
                 feq[s] = w[s] * (A + t * (T(1) + t * h));
             }
-            if constexpr (L::Q == 27) feq[0] += rho * (T(1.5) * T(a.dk1) * usq - T(a.dk0));
+            feq[0] += rho * (T(1.5) * T(a.dk1) * usq - T(a.dk0));
         } else {
 #pragma unroll
             for (int s = 0; s < Q; ++s) {
@@ -627,8 +628,8 @@ __device__ inline void node_relax_conserving(float (&f)[L::Q], const float (&feq
 }
 
 // Whether a step kernel relaxes fluid nodes with node_relax_conserving:
-// FP32 storage.  FP16's own rounding is stochastic, no bias; its D3Q27
-// float weights get their mass and momentum balance from the equilibrium
+// FP32 storage.  FP16's own rounding is stochastic, no bias; its float
+// weights get the mass and momentum balance of w from the equilibrium
 // corrections t3, dk0, dk1 instead, at a fraction of the cost (the
 // conserving relax took 13 % of the FP16 D3Q27 step).
 template <int ST, int Q>

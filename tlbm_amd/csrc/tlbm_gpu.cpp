@@ -65,11 +65,11 @@ void fill_weights(tlbm::StepArgs &a, bool exact, int storage) {
     a.k0 = float(sum_w - 1.0);
     a.k1 = float(1.0 + (sum_w - 1.0) - 3.0 * w2);
     a.k2 = float(1.0 - 3.0 * w2);
-    // FP16 D3Q27: equilibrium corrections from the float weights' balance to
-    // that of w (zero where the two coincide)
+    // FP16: equilibrium corrections from the float weights' balance to that
+    // of w (zero where the two coincide: the reference's D3Q15/D3Q19 weights)
     a.t3 = 3.0f;
     a.dk0 = a.dk1 = 0.0f;
-    if (storage == TLBM_STORAGE_F16 && L::Q == 27) {
+    if (storage == TLBM_STORAGE_F16) {
         double w2f = 0;
         for (int s = 0; s < L::Q; ++s) w2f += double(a.wf[s]) * double(L::ex[s] * L::ex[s]);
         const double k1f = 1.0 + (sum - 1.0) - 3.0 * w2f;
