@@ -9,18 +9,20 @@ tau = 0.55, starting from rest (f = w).
 
 A step is one fused stream-collide launch over the whole lattice with the
 populations resident in HBM.  The headline `value` is MLUPS with FP32
-storage (nx*ny*nz*steps / wall time of the timed steps); FP64 and FP16
-(f - w) storage run the same steps for the velocity L2 error and mass drift
-against FP64, reported under `variants`, after the warm-up and timed steps.
-With --full-steps 20000 FP32 and FP32X are also measured after the
-configuration's full 20000 steps on one x-row of the lattice (1x128x512: x
-is periodic and geometry and start are uniform in x, so the full lattice
-repeats that row exactly).  `roofline` prices the FP32 kernel
-at 2*Q*4 = 152 algorithmic bytes per lattice update against 8 TB/s, with
-its average duration from HIP events on the stream the kernel runs on.
+storage (nx*ny*nz*steps / wall time of the timed steps); FP64, FP16 (f - w)
+and FP32X storage run the same timed steps, reported under `variants`.
+After the timed region every storage runs the configuration's full 20000
+steps from rest on the full lattice (--accuracy-steps), and each reduced
+storage reports its velocity L2 over fluid nodes, its mass drift (FP64
+on-device sums) and the reference's own validation metric, the relative
+1-norm difference of the speed |u| over all nodes (python/validate.py:33-43),
+against the FP64 run.  `roofline` prices the FP32 kernel at 2*Q*4 = 152
+algorithmic bytes per lattice update against 8 TB/s, with its average
+duration from HIP events on the stream the kernel runs on.
 `cpu_baseline` times the reference's own MPI executable (built from its
-sources into oracle/_ref) on the host cores for a bounded number of steps
-of the same lattice, or the C oracle when that executable cannot run.
+sources into oracle/_ref) on the host's physical cores (those this process
+may run on, at most its CPU share) for 200 steps of the same lattice, or the
+C oracle when that executable cannot run.
 
     python bench.py [--gpus 1] [--steps 1000] [--warmup 100]
 """
@@ -63,43 +65,51 @@ def run_gpu(storage, nd, steps, warmup):
         t0 = time.perf_counter()
         ms_events = lat.step_timed(steps)
         wall = time.perf_counter() - t0
-        m = lat.macros()
-        red = lat.reduce()
     n = NX * NY * NZ
     return {
         "wall_s": wall,
         "kernel_ms": ms_events / steps,
         "mlups": n * steps / wall / 1e6,
-        "macros": m,
-        "mass": red["mass"],
     }
 
 
-def full_length_errors(storages, steps):
-    """Velocity L2 and mass drift against FP64 after `steps` steps on one
-    x-row (1 x NY x NZ) of the benchmark lattice, for the storages whose
-    arithmetic is deterministic per node (FP32, FP32X): every row of the
-    full lattice computes exactly what this row does.  FP16 is left out: its
-    stochastic rounding is independent across x in the full lattice, where
-    it largely averages out of the x-uniform flow, but one row makes it the
-    same at every x (20000 steps: 4.7e-3 on the row, 8.0e-4 on the lattice)."""
-    nd = geometry.channel(1, NY, NZ, walls=(), obstacles=[geometry.cylinder_x(64, 128, 12)])
+def accuracy_runs(nd, storages, steps):
+    """Each storage from rest over `steps` steps on the full lattice (one
+    lattice at a time), the reduced storages compared with FP64: velocity
+    L2 over fluid nodes, mass drift of the FP64 on-device sums, and the
+    relative 1-norm of the speed difference over all nodes."""
     fluid = nd == 0
     kw = dict(omega=1.0 / TAU, u_lbm=U_IN, rho_lbm=1.0)
+    ref = None
     out = {}
-    for st in ["f64"] + [s for s in storages if s in ("f32", "f32x")]:
-        with Lattice(Q, 1, NY, NZ, nd, storage=st, **kw) as lat:
+    for st in ["f64"] + [s for s in storages if s != "f64"]:
+        print(f"[bench] accuracy: {st}, {steps} steps", file=sys.stderr, flush=True)
+        with Lattice(Q, NX, NY, NZ, nd, storage=st, **kw) as lat:
             lat.step(steps)
-            out[st] = (lat.macros(), lat.reduce()["mass"])
-    m64, mass64 = out.pop("f64")
-    return {st: {"l2_velocity_vs_f64": l2_velocity(m, m64, fluid),
-                 "mass_drift_vs_f64": abs(mass - mass64) / mass64} for st, (m, mass) in out.items()}
+            m = lat.macros()
+            mass = lat.reduce()["mass"]
+        if st == "f64":
+            ref = (m, mass)
+            continue
+        out[st] = {"accuracy_steps": steps,
+                   "l2_velocity_vs_f64": l2_velocity(m, ref[0], fluid),
+                   "mass_drift_vs_f64": abs(mass - ref[1]) / ref[1],
+                   "rel_l1_speed_vs_f64": rel_l1_speed(m, ref[0])}
+    return out
 
 
 def l2_velocity(m, ref, fluid):
     a = np.stack([m[k][fluid] for k in ("ux", "uy", "uz")])
     b = np.stack([ref[k][fluid] for k in ("ux", "uy", "uz")])
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def rel_l1_speed(m, ref):
+    """The reference's validation metric (python/validate.py:33-43):
+    ||(|u| - |u_ref|)||_1 / || |u_ref| ||_1 over every node."""
+    s = np.sqrt(m["ux"] ** 2 + m["uy"] ** 2 + m["uz"] ** 2)
+    r = np.sqrt(ref["ux"] ** 2 + ref["uy"] ** 2 + ref["uz"] ** 2)
+    return float(np.sum(np.abs(s - r)) / np.sum(np.abs(r)))
 
 
 def traffic_bytes(storage):
@@ -149,20 +159,59 @@ def cpu_oracle(nd, steps):
     return NX * NY * NZ * steps / dt / 1e6
 
 
-def cpu_baseline(nd, args):
-    cores = args.cpu_cores
+def host_cores():
+    """(ranks, physical cores, logical CPUs, CPU model, share): the physical
+    cores among the CPUs this process may run on (lscpu's CORE/SOCKET of
+    each CPU in the affinity set), capped by the CPU share the host grants
+    one GPU job (OMP_NUM_THREADS when set)."""
+    aff = sorted(os.sched_getaffinity(0))
+    phys = set()
     try:
-        v = cpu_reference(nd, args.cpu_steps, cores, timeout=args.cpu_timeout)
-        return {"value": round(v, 3), "unit": "MLUPS", "cores": cores, "kind": "reference",
-                "sample": f"reference tLBMexec (real=float, MPICH, {cores} ranks, 1-D z-slabs) on "
-                          f"the full {NX}x{NY}x{NZ} D3Q19 lattice for {args.cpu_steps} steps; "
-                          "its own Estimated LPU/s (loop time, no dumps)"}
+        out = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True,
+                             timeout=10).stdout
+        allowed = set(aff)
+        for ln in out.splitlines():
+            if ln.startswith("#"):
+                continue
+            cpu, core, sock = (int(v) for v in ln.split(",")[:3])
+            if cpu in allowed:
+                phys.add((sock, core))
+    except (OSError, ValueError, subprocess.SubprocessError):
+        phys = set(aff)
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(aff)
+    ncores = max(1, len(phys))
+    return min(ncores, share), ncores, len(aff), model, share
+
+
+def cpu_baseline(nd, args):
+    ranks, ncores, nlogical, model, share = host_cores()
+    if args.cpu_cores > 0:
+        ranks = args.cpu_cores
+    where = (f"{model}; {ncores} physical cores / {nlogical} logical CPUs in this process's "
+             f"affinity set, CPU share {share}")
+    try:
+        v = cpu_reference(nd, args.cpu_steps, ranks, timeout=args.cpu_timeout)
+        return {"value": round(v, 3), "unit": "MLUPS", "cores": ranks, "kind": "reference",
+                "sample": f"reference tLBMexec (real=float, MPICH, {ranks} ranks, one per physical "
+                          f"core, 1-D z-slabs) on the full {NX}x{NY}x{NZ} D3Q19 lattice for "
+                          f"{args.cpu_steps} steps; its own Estimated LPU/s (loop time, no dumps); "
+                          f"host: {where}"}
     except Exception as exc:  # the reference cannot run here: time the oracle instead
-        steps = max(1, args.cpu_steps // 10)
+        steps = max(1, args.cpu_steps // 20)
         v = cpu_oracle(nd, steps)
         return {"value": round(v, 3), "unit": "MLUPS", "cores": 1, "kind": "port",
                 "sample": f"C oracle (FP32 restatement, 1 thread) on the full {NX}x{NY}x{NZ} "
-                          f"lattice for {steps} steps (reference run unavailable: {exc})"}
+                          f"lattice for {steps} steps (reference run unavailable: {exc}); "
+                          f"host: {where}"}
 
 
 def copy_calibration(mib, reps=5):
@@ -195,13 +244,14 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--full-steps", type=int, default=0,
-                    help="also run one x-row of the lattice this many steps (e.g. 20000, the "
-                         "configuration's length) for the L2 and mass drift against FP64; off by "
-                         "default, since its launches share the timed kernels' names in a trace")
-    ap.add_argument("--cpu-steps", type=int, default=20)
-    ap.add_argument("--cpu-cores", type=int, default=16)
-    ap.add_argument("--cpu-timeout", type=float, default=240.0)
+    ap.add_argument("--accuracy-steps", type=int, default=20000,
+                    help="after the timed region, run every storage this many steps from rest "
+                         "on the full lattice for its errors against FP64 (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=200)
+    ap.add_argument("--cpu-cores", type=int, default=0,
+                    help="MPI ranks of the CPU baseline (0: one per physical core, at most the "
+                         "host's CPU share)")
+    ap.add_argument("--cpu-timeout", type=float, default=300.0)
     ap.add_argument("--storages", default="f32,f64,f16,f32x",
                     help="storages to run; the first is the headline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -216,7 +266,6 @@ def main():
         sys.exit(2)
 
     nd = node_types()
-    fluid = nd == 0
     storages = args.storages.split(",")
     res = {}
     for st in storages:
@@ -229,18 +278,13 @@ def main():
     achieved = b_alg * n / (h["kernel_ms"] * 1e-3) / 1e9
     variants = {}
     for st, r in res.items():
-        v = {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 5),
-             "bytes_per_update": 2 * Q * BYTES_PER_VALUE[st],
-             "hbm_frac": round(2 * Q * BYTES_PER_VALUE[st] * n / (r["kernel_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
-        if "f64" in res and st != "f64":
-            v["l2_velocity_vs_f64"] = l2_velocity(r["macros"], res["f64"]["macros"], fluid)
-            v["mass_drift_vs_f64"] = abs(r["mass"] - res["f64"]["mass"]) / res["f64"]["mass"]
-        variants[st] = v
-    if args.full_steps > 0 and "f64" in res:
-        print(f"[bench] x-row accuracy run: {args.full_steps} steps", file=sys.stderr, flush=True)
-        for st, e in full_length_errors(storages, args.full_steps).items():
-            variants[st][f"l2_velocity_vs_f64_{args.full_steps}_steps"] = e["l2_velocity_vs_f64"]
-            variants[st][f"mass_drift_vs_f64_{args.full_steps}_steps"] = e["mass_drift_vs_f64"]
+        variants[st] = {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 5),
+                        "bytes_per_update": 2 * Q * BYTES_PER_VALUE[st],
+                        "hbm_frac": round(2 * Q * BYTES_PER_VALUE[st] * n / (r["kernel_ms"] * 1e-3)
+                                          / 1e9 / PEAK_HBM_GBS, 4)}
+    if args.accuracy_steps > 0:
+        for st, e in accuracy_runs(nd, storages, args.accuracy_steps).items():
+            variants[st].update(e)
     line = {
         "metric": METRIC,
         "value": round(h["mlups"], 1),

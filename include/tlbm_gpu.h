@@ -26,6 +26,7 @@ extern "C" {
 #define TLBM_EHIP (-2)     /* a HIP runtime call failed */
 #define TLBM_ENOMEM (-3)   /* device or host allocation failed */
 #define TLBM_ESTATE (-4)   /* call not valid in the handle's current state */
+#define TLBM_ENONFINITE (-5) /* the state holds NaN or infinite populations (tlbm_gpu_reduce) */
 
 /* Population storage in HBM.  F64 stores f_s and computes in FP64 with the
  * reference's operation order.  F32 stores d_s = f_s - w_s and F16 the
@@ -111,8 +112,11 @@ int tlbm_gpu_init_from_macros(tlbm_gpu *h, const double *ux, const double *uy, c
 /* FP64 reductions over the populations the next step will read:
  * out[0] = mass sum f, out[1..3] = momentum sum f*e, out[4] = relative
  * change of the state over the last step, ||B_n - B_{n-1}|| / ||B_n||
- * (0 before the first step).  No reference counterpart: the reference has
- * no conservation or residual checks. */
+ * (0 before the first step).  The same pass counts populations that are
+ * NaN or infinite: when there are any, out[] is still filled and the call
+ * returns TLBM_ENONFINITE with the count in the message (a diverged run).
+ * No reference counterpart: the reference has no conservation, residual
+ * or divergence checks (its loop, src/tLBM.cpp:45-62, steps on). */
 int tlbm_gpu_reduce(tlbm_gpu *h, double out[5]);
 
 /* enable = 1: zero the time-average sums and from then on add ux, uy, uz

@@ -85,41 +85,87 @@ def test_correction_removes_a_residue(q):
         assert abs(left) <= Fr(1, 10 ** 7) * max(abs(x) for x in m)
 
 
-def test_fp16_d3q27_equilibrium_correction():
-    """FP16 D3Q27 builds its equilibrium from the float weights w_f and
-    corrects it to the balance of the double weights w: 3 e.u becomes
-    t3 e.u with t3 = 3 W2 / W2_f, and the rest population gains
-    rho (1.5 dk1 u^2 - dk0) with dk0 = k0_f - k0, dk1 = k1_f - k1.  In exact
-    arithmetic the momentum of f - feq is then k2 rho u exactly, and its
-    mass rho (1.5 k1 u^2 - k0) up to the t3^2 change of the quadratic term
-    (4.5 rho W2_f u^2 (c^2 - 1), c = W2 / W2_f: about 1e-11 here, against
-    the 7.45e-9 that the correction removes)."""
-    e, w = _weights(27, "double")
-    _, wf = _weights(27, "float")
-    q = 27
+def _host_fp16_constants(w, wf, e, q):
+    """dt3, dk0, dk1 as the host computes them (tlbm_gpu.cpp fill_weights):
+    double arithmetic on the weights w and their float values wf, each
+    constant then rounded to float, as the kernel receives it."""
+    wd = [float(x) for x in w]
+    wfd = [float(x) for x in wf]
+    sum_w, sum_f = 0.0, 0.0
+    w2, w2f = 0.0, 0.0
+    for s in range(q):
+        sum_w += wd[s]
+        sum_f += wfd[s]
+        w2 += wd[s] * float(int(e[s, 0]) ** 2)
+        w2f += wfd[s] * float(int(e[s, 0]) ** 2)
+    k1f = 1.0 + (sum_f - 1.0) - 3.0 * w2f
+    dt3 = np.float32(3.0 * ((w2 - w2f) / w2f))
+    dk0 = np.float32((sum_f - 1.0) - (sum_w - 1.0))
+    dk1 = np.float32(k1f - (1.0 + (sum_w - 1.0) - 3.0 * w2))
+    return Fr(float(dt3)), Fr(float(dk0)), Fr(float(dk1))
+
+
+def _exact_weight_tables(q):
+    e, _, _, w_exact = lattice_tables.tables(q)
+    return e, [Fr(float(x)) for x in w_exact], [Fr(float(np.float32(x))) for x in w_exact]
+
+
+@pytest.mark.parametrize("case", ["d3q27", "d3q15_exact", "d3q19_exact"])
+def test_fp16_equilibrium_correction(case):
+    """FP16 builds its equilibrium from the float weights w_f and corrects it
+    to the balance of the weights w (D3Q27's double weights; the exact
+    D3Q15/D3Q19 fractions): 3 e.u becomes (3 + dt3) e.u with
+    dt3 = 3 (W2 / W2_f - 1), and the rest population gains
+    rho (1.5 dk1 u^2 - dk0) with dk0 = k0_f - k0, dk1 = k1_f - k1.  The
+    constants are the float values the host sends (3 + dt3 itself would
+    round to 3.0f).  In exact arithmetic the momentum of f - feq is then
+    k2 rho u up to the float rounding of dt3 (W2_f |dt3 - float(dt3)| rho u,
+    below 1e-15 rho u), against 7.45e-9 rho u without dt3; its mass is
+    rho (1.5 k1 u^2 - k0) up to the (1 + dt3/3)^2 change of the quadratic
+    term (about 1e-11 here, against the 7.45e-9 that dk0 removes)."""
+    if case == "d3q27":
+        q = 27
+        e, w = _weights(27, "double")
+        _, wf = _weights(27, "float")
+    else:
+        q = 15 if case == "d3q15_exact" else 19
+        e, w, wf = _exact_weight_tables(q)
     k0, k0f = sum(w) - 1, sum(wf) - 1
     W2 = sum(w[s] * int(e[s, 0]) ** 2 for s in range(q))
     W2f = sum(wf[s] * int(e[s, 0]) ** 2 for s in range(q))
-    k1, k1f, k2 = 1 + k0 - 3 * W2, 1 + k0f - 3 * W2f, 1 - 3 * W2
-    c = W2 / W2f
-    dk0, dk1 = k0f - k0, k1f - k1
-    assert abs(float(dk0) - 7.450580596923828e-09) < 1e-15
-    rng = np.random.default_rng(27)
+    k1, k2, k2f = 1 + k0 - 3 * W2, 1 - 3 * W2, 1 - 3 * W2f
+    dt3, dk0, dk1 = _host_fp16_constants(w, wf, e, q)
+    assert dt3 != 0 and abs(float(dt3) + 2.2e-8) < 0.2e-8
+    assert float(3 + dt3) != float(np.float32(3 + dt3))  # the scale alone rounds to 3.0f
+    c = (3 + dt3) / 3
+    rng = np.random.default_rng(q)
     for trial in range(3):
         f = [wf[s] * (1 + Fr(float(rng.uniform(-0.2, 0.2)))) for s in range(q)]
         rho = sum(f)
         u = [sum(f[s] * int(e[s, a]) for s in range(q)) / rho for a in range(3)]
         usq = sum(x * x for x in u)
-        feq = []
-        for s in range(q):
-            eu = c * sum(int(e[s, a]) * u[a] for a in range(3))
-            feq.append(wf[s] * rho * (1 + 3 * eu + Fr(9, 2) * eu * eu - Fr(3, 2) * usq))
-        feq[0] += rho * (Fr(3, 2) * dk1 * usq - dk0)
-        nq = [f[s] - feq[s] for s in range(q)]
+
+        def nonequilibrium(scale, with_rest):
+            feq = []
+            for s in range(q):
+                eu = scale * sum(int(e[s, a]) * u[a] for a in range(3))
+                feq.append(wf[s] * rho * (1 + 3 * eu + Fr(9, 2) * eu * eu - Fr(3, 2) * usq))
+            if with_rest:
+                feq[0] += rho * (Fr(3, 2) * dk1 * usq - dk0)
+            return [f[s] - feq[s] for s in range(q)]
+
+        nq = nonequilibrium(c, True)
         for a in range(3):
-            assert sum(nq[s] * int(e[s, a]) for s in range(q)) == k2 * rho * u[a]
+            mom = sum(nq[s] * int(e[s, a]) for s in range(q))
+            assert abs(mom - k2 * rho * u[a]) <= Fr(1, 10 ** 15) * abs(rho * u[a])
         target = rho * (Fr(3, 2) * k1 * usq - k0)
-        bound = Fr(9, 2) * rho * W2f * usq * abs(c * c - 1) * Fr(101, 100)
+        bound = Fr(9, 2) * rho * W2f * usq * abs(c * c - 1) * Fr(101, 100) + \
+            Fr(1, 10 ** 15) * rho
         assert abs(sum(nq) - target) <= bound
-        # without the correction the mass balance misses by ~rho dk0
+        # without the correction both balances miss by ~7.45e-9 rho (u)
         assert abs(float(dk0)) > 100 * float(bound)
+        plain = nonequilibrium(Fr(1), False)
+        for a in range(3):
+            mom = sum(plain[s] * int(e[s, a]) for s in range(q))
+            assert mom == k2f * rho * u[a]
+            assert abs(mom - k2 * rho * u[a]) > Fr(5, 10 ** 9) * abs(rho * u[a])

@@ -153,3 +153,49 @@ def test_stale_ndtype_sidecar_is_ignored(tmp_path):
     os.utime(os.path.join(d, "ndType.lbm"), (t + 10, t + 10))  # the text file is newer
     r = run_exec(d)
     assert r.returncode == 1 and "ndType.lbm: node 7 has type 4" in r.stderr
+
+
+@pytest.fixture(scope="module")
+def stub_exec(tmp_path_factory):
+    """The driver's host sources linked against the C-ABI test double
+    (tests/cabi_stub.cpp) instead of the GPU library: the driver's control
+    flow on a machine without a GPU."""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    host = os.path.join(root, "tlbm_amd", "host")
+    exe = str(tmp_path_factory.mktemp("stub") / "tlbm_exec_stub")
+    srcs = [os.path.join(host, f) for f in ("Problem.cpp", "TLBM_Partition.cpp", "h5_io.cpp",
+                                             "tlbm_main.cpp")]
+    srcs.append(os.path.join(root, "tests", "cabi_stub.cpp"))
+    subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(root, "include"), "-I", host,
+                    "-I", os.path.join(root, "tlbm_amd", "csrc"), *srcs, "-o", exe], check=True)
+    return exe
+
+
+def _run_stub(exe, d, env):
+    e = dict(os.environ)
+    e.update(env)
+    return subprocess.run([exe], cwd=d, capture_output=True, text=True, env=e, timeout=60)
+
+
+def test_driver_stops_when_the_state_is_not_finite(tmp_path, stub_exec):
+    """Divergence guard: the driver checks the state every tsRepFreq steps
+    and, when the library reports NaN or infinite populations, stops with a
+    non-zero status naming the time step (here the library reports them
+    from step 12 on; the first check after that is at step 15)."""
+    p = good_params(numTs=40, tsRepFreq=5, plotFreq=100)
+    d = case_dir(tmp_path, p, nd_for(p))
+    r = _run_stub(stub_exec, d, {"TLBM_STUB_NONFINITE_FROM": "12"})
+    assert r.returncode == 1
+    assert "time step 15:" in r.stderr and "NaN or infinite" in r.stderr
+    assert "Executing time step: 15" in r.stdout and "Executing time step: 20" not in r.stdout
+    assert "Test complete." not in r.stdout
+
+
+def test_driver_guard_can_be_switched_off(tmp_path, stub_exec):
+    p = good_params(numTs=40, tsRepFreq=5, plotFreq=100)
+    d = case_dir(tmp_path, p, nd_for(p))
+    r = _run_stub(stub_exec, d, {"TLBM_STUB_NONFINITE_FROM": "12", "TLBM_CHECK_FINITE": "0"})
+    assert r.returncode == 0, r.stderr
+    assert "Test complete." in r.stdout
+    r = _run_stub(stub_exec, d, {})
+    assert r.returncode == 0 and "Executing time step: 40" in r.stdout

@@ -1,0 +1,218 @@
+"""The configurations at their full sizes and lengths on the GPU, and the
+divergence guard.
+
+* D3Q19 cylinder array 128x128x512 (BGK, tau 0.55, inlet u_z 0.04) over its
+  full 20000 steps on the full lattice: FP32, FP16 (f - w)/w and FP32X
+  against FP64 (velocity L2 over fluid nodes, mass drift from the FP64
+  on-device sums).
+* D3Q27 Smagorinsky porous channel 384^3 (cs 0.1, tau 0.505, seeded sphere
+  pack to porosity 0.65, inlet ramped over 3000 steps): FP32 against FP64
+  to step 2000, the window in which the flow stays in the weakly
+  compressible regime.
+* D3Q19 1024x512x512 prolate spheroid: FP16 + the closure network and FP32
+  Smagorinsky for 1000 steps from uniform flow (finite state, drag), and
+  FP32 / FP16 Smagorinsky against FP64 on the 2x-downsampled 512x256x256
+  copy.
+* FP16's equilibrium correction keeps the momentum of the double weights
+  (periodic box, uniform flow), and NaN populations are reported by the
+  FP64 reductions.
+"""
+import numpy as np
+import pytest
+
+from tlbm_amd import geometry, workloads
+from tlbm_amd.closure import uniform_start
+from tlbm_amd.gpu import ENONFINITE, Lattice, TlbmError
+
+pytestmark = pytest.mark.gpu
+
+
+def _log(msg):
+    print(msg, flush=True)
+
+
+def _l2_streamed(lat_a, lat_b, mask):
+    """Relative L2 of u over `mask` between two lattices, one velocity
+    component in memory at a time."""
+    num = den = 0.0
+    ma = lat_a.macros()
+    mb = lat_b.macros()
+    for k in ("ux", "uy", "uz"):
+        a, b = ma.pop(k)[mask], mb.pop(k)[mask]
+        num += float(np.sum((a - b) ** 2))
+        den += float(np.sum(b * b))
+        del a, b
+    return float(np.sqrt(num / den))
+
+
+@pytest.mark.timeout(300)
+def test_cylinder_array_full_length(gpu_lib):
+    """All 20000 steps of the benchmark configuration on the full lattice:
+    FP32 within 1e-5 of FP64, FP16 within 1e-3, FP32X within 1e-5; mass
+    drift of each against FP64 below 1e-6."""
+    _log("cylinder array 128x128x512, 20000 steps: f64 f32 f16 f32x")
+    rep = workloads.run_cylinders(steps=20000, chunk=5000, checkpoints=(1100, 20000))
+    last = rep["checkpoints"][-1]
+    assert last["step"] == 20000
+    _log(f"step 20000: {last}")
+    assert last["l2_velocity_f32_vs_f64"] <= 1e-5
+    assert last["l2_velocity_f16_vs_f64"] <= 1e-3
+    assert last["l2_velocity_f32x_vs_f64"] <= 1e-5
+    for st in ("f32", "f16", "f32x"):
+        assert last[f"mass_drift_{st}_vs_f64"] <= 1e-6, st
+    drag = last["drag"]
+    assert drag["f64"] > 0
+    assert abs(drag["f32"] / drag["f64"] - 1) < 1e-4
+    assert abs(drag["f16"] / drag["f64"] - 1) < 1e-2
+
+
+@pytest.mark.timeout(400)
+def test_porous_384_fp32_follows_fp64(gpu_lib):
+    """The D3Q27 porous channel at 384^3 (seed 20251202): FP32 within 1e-5
+    of FP64 in velocity and 1e-7 in mass at steps 1000 and 2000."""
+    _log("porous D3Q27 384^3: geometry, then f64 + f32 to step 2000")
+    rep = workloads.run_porous(n=384, steps=2000, seed=20251202, ramp=3000,
+                               checkpoints=(1000, 2000))
+    assert rep["spheres"] == 8426 and abs(rep["porosity"] - 0.65) < 1e-3
+    assert [c["step"] for c in rep["checkpoints"]] == [1000, 2000]
+    for cp in rep["checkpoints"]:
+        _log(f"porous step {cp['step']}: {cp}")
+        assert cp["l2_velocity_vs_f64"] <= 1e-5, cp
+        assert cp["mass_drift_vs_f64"] <= 1e-7, cp
+    for st in ("f64", "f32"):
+        assert rep["runs"][st]["mlups"] > 0
+
+
+@pytest.mark.timeout(400)
+def test_spheroid_full_lattice(gpu_lib):
+    """1024x512x512 (268M nodes): FP16 + network (the shipped network at its
+    calibrated base relaxation time) and FP32 Smagorinsky, 1000 steps each
+    from uniform flow.  Both states stay finite (the FP64 reduction counts
+    NaN and infinite populations), the drag on the spheroid is positive and
+    the two runs agree on it within 10 %, and their velocity fields within
+    10 % (they differ by their closure)."""
+    from tlbm_amd.closure import load_network, split_closure
+    dims, nd, body = workloads.spheroid_case(1)
+    nx, ny, nz = dims
+    assert dims == (1024, 512, 512)
+    fluid = nd != geometry.SOLID
+    net, offset = split_closure(load_network(workloads.DEFAULT_NETWORK))
+    start = uniform_start(nd, 0.05)
+    res = {}
+    lats = {}
+    try:
+        for key, storage, cl in (("f16_network", "f16", net), ("f32_smagorinsky", "f32", None)):
+            _log(f"spheroid 1024x512x512 {key}: 1000 steps")
+            tau = 0.51 + (offset if cl is not None else 0.0)
+            lat = Lattice(19, nx, ny, nz, nd, storage=storage, omega=1.0 / tau, u_lbm=0.05,
+                          cs=0.0 if cl is not None else 0.1)
+            lats[key] = lat
+            if cl is not None:
+                lat.set_closure(*cl)
+            lat.init_from_macros(*start)
+            ms = lat.step_timed(1000)
+            red = lat.reduce()
+            assert red["finite"]
+            drag = float(np.sum(lat.forces()["Fz"][body]))
+            res[key] = {"ms_per_step": ms / 1000, "drag": drag, "mass": red["mass"]}
+            _log(f"{key}: {res[key]}")
+        l2 = _l2_streamed(lats["f16_network"], lats["f32_smagorinsky"], fluid)
+    finally:
+        for lat in lats.values():
+            lat.close()
+    _log(f"velocity L2 f16+network vs f32 smagorinsky: {l2:.3e}")
+    d16, d32 = res["f16_network"]["drag"], res["f32_smagorinsky"]["drag"]
+    assert d32 > 0 and d16 > 0
+    assert abs(d16 / d32 - 1) < 0.1
+    assert l2 < 0.1
+
+
+@pytest.mark.timeout(300)
+def test_spheroid_downsampled_storages_vs_fp64(gpu_lib):
+    """The 512x256x256 copy (semi-axes 48/16, tau 0.505), Smagorinsky cs
+    0.1, 1000 steps from uniform flow: FP32 and FP16 against FP64 in
+    velocity, mass and drag."""
+    dims, nd, body = workloads.spheroid_case(2)
+    nx, ny, nz = dims
+    fluid = nd != geometry.SOLID
+    start = uniform_start(nd, 0.05)
+    lats = {}
+    out = {}
+    try:
+        for st in ("f64", "f32", "f16"):
+            _log(f"spheroid 512x256x256 {st}: 1000 steps")
+            lat = Lattice(19, nx, ny, nz, nd, storage=st, omega=1.0 / 0.505, u_lbm=0.05, cs=0.1)
+            lats[st] = lat
+            lat.init_from_macros(*start)
+            lat.step(1000)
+            red = lat.reduce()
+            out[st] = {"mass": red["mass"], "drag": float(np.sum(lat.forces()["Fz"][body]))}
+        for st in ("f32", "f16"):
+            out[st]["l2"] = _l2_streamed(lats[st], lats["f64"], fluid)
+            out[st]["mass_drift"] = abs(out[st]["mass"] / out["f64"]["mass"] - 1)
+            out[st]["drag_rel"] = abs(out[st]["drag"] / out["f64"]["drag"] - 1)
+    finally:
+        for lat in lats.values():
+            lat.close()
+    _log(f"downsampled spheroid: {out}")
+    assert out["f32"]["l2"] <= 1e-4 and out["f32"]["mass_drift"] <= 1e-6
+    assert out["f16"]["l2"] <= 1e-2 and out["f16"]["mass_drift"] <= 1e-5
+    assert out["f32"]["drag_rel"] <= 1e-3 and out["f16"]["drag_rel"] <= 1e-2
+
+
+@pytest.mark.parametrize("q,exact", [(27, False), (15, True), (19, True)])
+def test_fp16_momentum_balance_uniform_flow(gpu_lib, q, exact):
+    """A periodic 128^3 box in uniform flow u = (0.02, -0.01, 0.05), 4000
+    steps at omega 1.7.  The FP64 run keeps its momentum to rounding.  FP16
+    builds its equilibrium from the float weights, whose second moment
+    differs from that of the weights used (D3Q27's double weights, the exact
+    D3Q15/D3Q19 fractions) by 7.45e-9 relative: uncorrected that is a
+    momentum source of omega * 7.45e-9 of the momentum per step, 5.1e-5
+    after 4000 steps.  With the correction (3 + dt3) e.u the drift is the
+    random walk of FP16's stochastic rounding, about 1e-6 here."""
+    n = 128
+    nd = geometry.periodic_box(n, n, n)
+    N = nd.size
+    u = (0.02, -0.01, 0.05)
+    mac = [np.full(N, u[0]), np.full(N, u[1]), np.full(N, u[2]), np.ones(N)]
+    mom = {}
+    for st in ("f64", "f16"):
+        with Lattice(q, n, n, n, nd, storage=st, omega=1.7, exact_weights=exact) as lat:
+            lat.init_from_macros(*mac)
+            lat.step(1)
+            p0 = lat.reduce()["momentum"]
+            lat.step(4000)
+            red = lat.reduce()
+            mom[st] = (p0, red["momentum"], red["mass"])
+    p0, p1, _ = mom["f64"]
+    assert np.max(np.abs(p1 / p0 - 1)) <= 1e-12
+    p0, p1, _ = mom["f16"]
+    drift = np.abs(p1 / p0 - 1)
+    _log(f"q={q} exact={exact} f16 momentum drift per component {drift}")
+    assert np.max(drift) <= 1e-5, drift
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16", "f32x"])
+def test_nonfinite_population_is_reported(gpu_lib, storage):
+    """One NaN population set through the C-ABI: the FP64 reduction reports
+    TLBM_ENONFINITE with the count (and still returns the sums when asked);
+    after steps the NaN has spread and is still reported.  A finite state
+    reports nothing."""
+    nx, ny, nz = 16, 12, 20
+    nd = geometry.channel(nx, ny, nz, obstacles=[geometry.sphere(8, 6, 8, 3)])
+    with Lattice(19, nx, ny, nz, nd, storage=storage, omega=1.6, u_lbm=0.05) as lat:
+        lat.step(10)
+        assert lat.reduce()["finite"]
+        f = lat.populations()
+        f[nx * ny * 9 + nx * 6 + 4, 7] = np.nan
+        lat.set_populations(f)
+        with pytest.raises(TlbmError) as ei:
+            lat.reduce()
+        assert ei.value.code == ENONFINITE
+        assert "1 populations are NaN or infinite" in str(ei.value)
+        r = lat.reduce(allow_nonfinite=True)
+        assert not r["finite"] and np.isnan(r["mass"])
+        lat.step(5)
+        with pytest.raises(TlbmError) as ei:
+            lat.reduce()
+        assert ei.value.code == ENONFINITE and "diverged" in str(ei.value)

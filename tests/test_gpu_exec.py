@@ -178,3 +178,28 @@ def test_exec_reads_the_binary_node_type_sidecar(gpu_lib, tmp_path):
     got = read_dump(d, 0, np.float64)
     for f, ref in case.out.items():
         assert rel_err(got[f], ref) <= TOL64, f
+
+
+def test_exec_stops_when_the_state_is_not_finite(gpu_lib, tmp_path):
+    """Divergence guard end to end: a restart.h5 with one NaN velocity
+    (written by the restart tool from a dump) starts the run from a NaN
+    equilibrium at that node; the driver's check after step 10 (tsRepFreq)
+    finds it, and the driver stops with status 1 naming the step."""
+    nx, ny, nz = 8, 7, 12
+    nd = geometry.channel(nx, ny, nz, obstacles=[geometry.sphere(3, 3, 4, 2)])
+    kw = dict(latticeType="D3Q19", tsRepFreq=10, uLBM=0.05, omega=1.6, nx=nx, ny=ny, nz=nz)
+    d = case_dir(tmp_path, io_formats.Params(numTs=10, plotFreq=10, **kw), nd)
+    r = run_exec(d, storage="f64")
+    assert r.returncode == 0, r.stderr
+    ux = read_raw(d, "ux0.b_dat", np.float64)
+    ux[nx * ny * 6 + nx * 3 + 4] = np.nan
+    ux.tofile(os.path.join(d, "ux0.b_dat"))
+    rr = subprocess.run([require(RESTART), "0", "f64"], cwd=d, capture_output=True, text=True)
+    assert rr.returncode == 0, rr.stderr
+    io_formats.write_params(os.path.join(d, "params.lbm"),
+                            io_formats.Params(numTs=40, plotFreq=40, restartFlag=1, **kw))
+    for storage in ("f64", "f32", "f16"):
+        r = run_exec(d, storage=storage)
+        assert r.returncode == 1, (storage, r.stdout, r.stderr)
+        assert "time step 10:" in r.stderr and "NaN or infinite" in r.stderr, storage
+        assert "Test complete." not in r.stdout

@@ -230,18 +230,20 @@ __device__ inline void block_sums(double (&acc)[kRed]) {
 }
 
 // Per-workgroup FP64 partial sums over the state: sum f, sum f e_x,
-// sum f e_y, sum f e_z, sum (B_n - B_{n-1})^2, sum B_n^2 (kRed values).
+// sum f e_y, sum f e_z, sum (B_n - B_{n-1})^2, sum B_n^2 and the number of
+// populations that are NaN or infinite (kRed values).
 template <class L, int ST>
 __global__ void __launch_bounds__(kBlock)
 k_reduce_partial(const typename StorageT<ST>::st *__restrict__ cur,
                  const typename StorageT<ST>::st *__restrict__ prev, double *__restrict__ partial,
                  StepArgs a) {
     using S = StorageT<ST>;
-    double acc[kRed] = {0, 0, 0, 0, 0, 0};
+    double acc[kRed] = {0, 0, 0, 0, 0, 0, 0};
     for (uint32_t g = blockIdx.x * kBlock + threadIdx.x; g < a.N; g += gridDim.x * kBlock) {
 #pragma unroll
         for (int s = 0; s < L::Q; ++s) {
             const double v = S::absolute(S::load(cur + fidx<L::Q>(s, g, a), s, a), s, a);
+            if (!__builtin_isfinite(v)) acc[6] += 1.0;
             acc[0] += v;
             acc[1] += L::ex[s] * v;
             acc[2] += L::ey[s] * v;
@@ -262,7 +264,7 @@ k_reduce_partial(const typename StorageT<ST>::st *__restrict__ cur,
 template <int ST>
 __global__ void __launch_bounds__(kBlock)
 k_reduce_final(const double *__restrict__ partial, int nparts, double *__restrict__ out) {
-    double acc[kRed] = {0, 0, 0, 0, 0, 0};
+    double acc[kRed] = {0, 0, 0, 0, 0, 0, 0};
     for (int i = threadIdx.x; i < nparts; i += kBlock) {
 #pragma unroll
         for (int k = 0; k < kRed; ++k) acc[k] += partial[size_t(i) * kRed + k];

@@ -45,7 +45,7 @@ enum Storage : int { ST_F64 = 0, ST_F32 = 1, ST_F16 = 2, ST_F32X = 3 };
 enum Mode : int { MODE_STEP = 0, MODE_MACROS = 1, MODE_HEAD = 2, MODE_AVG = 3, MODE_STRESS = 4 };
 
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
-constexpr int kRed = 6;      // values per FP64 reduction partial
+constexpr int kRed = 7;      // values per FP64 reduction partial
 constexpr int kFilterPlanes = 16;  // box filter output planes
 constexpr int kClosureBlock = 512;  // workgroup of the closure collide
 // non-temporal population loads in the closure collide: FP16 only (FP32:
@@ -78,10 +78,12 @@ struct StepArgs {
     double dw_d, c_in_d, c_out_d;
     // FP16: equilibrium corrections that give the float weights the
     // exact-arithmetic mass and momentum balance of the weights w (D3Q27 and
-    // exact weights): 3 e.u is scaled to t3 e.u (t3 = 3 W2 / W2_f) and the
-    // rest population gains rho (1.5 dk1 u^2 - dk0); t3 = 3 and
-    // dk0 = dk1 = 0 where the two coincide and in the other storages
-    float t3, dk0, dk1;
+    // exact weights): 3 e.u is scaled to (3 + dt3) e.u (dt3 = 3 (W2 / W2_f - 1),
+    // about -2.2e-8: 3 + dt3 itself rounds to 3 in float, so the kernel
+    // adds dt3 rho u to the unrounded product 3 rho u in one FMA) and the
+    // rest population gains rho (1.5 dk1 u^2 - dk0); dt3 = dk0 = dk1 = 0
+    // where the two coincide and in the other storages
+    float dt3, dk0, dk1;
     // exact-arithmetic moments of f - feq with the weights w (the conserving
     // relax; its float weights may sum differently): sum_s (f - feq) =
     // rho (1.5 k1 u^2 - k0), sum_s e (f - feq) = k2 rho u; k0 = sum_s w - 1,
@@ -392,6 +394,10 @@ struct PushOut {
     }
 };
 
+// a * b + c rounded once.
+__device__ inline float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // sum_k e_k u_k over the nonzero components, in x, y, z order.
 template <class T>
 __device__ inline T edot(int ex, int ey, int ez, T ux, T uy, T uz) {
@@ -519,9 +525,17 @@ __device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho,
             // rewritten with t = e.(3 rho u) = rho cu:  w (A + t + t^2 / (2 rho)),
             // A = delta - 3/2 rho u^2 (the same polynomial, fewer operations)
             const T A = dlt - T(1.5) * rho * usq;
-            const T h = T(0.5) * __builtin_amdgcn_rcpf(rho);
-            const T t3 = T(a.t3);
-            const T Ux = t3 * rho * ux, Uy = t3 * rho * uy, Uz = t3 * rho * uz;
+            T h;
+            if constexpr (sizeof(T) == 8)
+                h = T(0.5) / rho;  // FP32X: every operation in FP64
+            else
+                h = T(0.5) * __builtin_amdgcn_rcpf(rho);
+            // U = (3 + dt3) rho u: fma(3 rho, u, dt3 rho u) rounds the
+            // exact product plus the correction once, so the correction
+            // survives rounding on average; with dt3 = 0 it is 3 rho * u
+            // bit for bit (the addend is a zero of u's sign)
+            const T r3 = T(3) * rho, rd = T(a.dt3) * rho;
+            const T Ux = fma_t(r3, ux, rd * ux), Uy = fma_t(r3, uy, rd * uy), Uz = fma_t(r3, uz, rd * uz);
 #pragma unroll
             for (int s = 0; s < Q; ++s) {
                 const T t = (s == 0) ? T(0) : edot<T>(L::ex[s], L::ey[s], L::ez[s], Ux, Uy, Uz);
@@ -630,7 +644,7 @@ __device__ inline void node_relax_conserving(float (&f)[L::Q], const float (&feq
 // Whether a step kernel relaxes fluid nodes with node_relax_conserving:
 // FP32 storage.  FP16's own rounding is stochastic, no bias; its float
 // weights get the mass and momentum balance of w from the equilibrium
-// corrections t3, dk0, dk1 instead, at a fraction of the cost (the
+// corrections dt3, dk0, dk1 instead, at a fraction of the cost (the
 // conserving relax took 13 % of the FP16 D3Q27 step).
 template <int ST, int Q>
 constexpr bool kConserve = ST == ST_F32;

@@ -67,13 +67,15 @@ void fill_weights(tlbm::StepArgs &a, bool exact, int storage) {
     a.k2 = float(1.0 - 3.0 * w2);
     // FP16: equilibrium corrections from the float weights' balance to that
     // of w (zero where the two coincide: the reference's D3Q15/D3Q19 weights)
-    a.t3 = 3.0f;
+    a.dt3 = 0.0f;
     a.dk0 = a.dk1 = 0.0f;
     if (storage == TLBM_STORAGE_F16) {
         double w2f = 0;
         for (int s = 0; s < L::Q; ++s) w2f += double(a.wf[s]) * double(L::ex[s] * L::ex[s]);
         const double k1f = 1.0 + (sum - 1.0) - 3.0 * w2f;
-        a.t3 = float(3.0 * w2 / w2f);
+        // the deviation of the scale 3 W2 / W2_f from 3 (it fits a float;
+        // the scale itself rounds to 3.0f)
+        a.dt3 = float(3.0 * ((w2 - w2f) / w2f));
         a.dk0 = float((sum - 1.0) - (sum_w - 1.0));
         a.dk1 = float(k1f - (1.0 + (sum_w - 1.0) - 3.0 * w2));
     }
@@ -501,6 +503,13 @@ int tlbm_gpu_reduce(tlbm_gpu *h, double out[5]) {
     out[2] = r[2];
     out[3] = r[3];
     out[4] = (prev && r[5] > 0.0) ? std::sqrt(r[4] / r[5]) : 0.0;
+    if (r[6] > 0.0) {
+        char buf[160];
+        std::snprintf(buf, sizeof buf,
+                      "%.0f populations are NaN or infinite after %lld steps: the run diverged",
+                      r[6], (long long)h->step_count);
+        return fail(TLBM_ENONFINITE, buf);
+    }
     return TLBM_OK;
 }
 

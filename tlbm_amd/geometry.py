@@ -119,40 +119,63 @@ def rsa_spheres(nx: int, ny: int, nz: int, r_min: float, r_max: float, porosity:
     rng = np.random.default_rng(seed)
     cell = 2.0 * r_max
     ncx, ncy, ncz = (max(1, int(np.ceil(n / cell))) for n in (nx, ny, nz))
-    cells: dict = {}
-    centres, radii = [], []
+    # cell list as a dense array, padded by one cell on every side so that
+    # the 27 cells around any cell are one slice; K slots per cell (no two
+    # accepted centres are closer than 2 r_min, so a cell of edge 2 r_max
+    # holds at most ~(r_max / r_min + 1)^3 of them)
+    K = int(np.ceil((r_max / r_min + 1.0) ** 3)) + 1
+    grid = np.full((ncx + 2, ncy + 2, ncz + 2, K), -1, np.int64)
+    fill = np.zeros((ncx + 2, ncy + 2, ncz + 2), np.int64)
+    cap = 1024
+    C = np.empty((cap, 3), np.float64)
+    R = np.empty(cap, np.float64)
+    n_acc = 0
     target = (1.0 - porosity) * nx * ny * nz
     solid = 0.0 if solid_mask is None else float(solid_mask.sum())
     attempts = 0
+    # the generator's doubles are drawn in batches and mapped exactly as
+    # Generator.uniform maps them (low + (high - low) * u, one double for the
+    # radius, then three for the centre), so the sequence is the one
+    # per-candidate uniform() calls give
+    batch = np.empty(0)
+    pos = 0
+    lo_c = np.zeros(3)
     while solid < target and attempts < max_attempts:
+        if pos + 4 > batch.size:
+            batch = rng.random(4 * 4096)
+            pos = 0
         attempts += 1
-        r = rng.uniform(r_min, r_max)
+        u = batch[pos:pos + 4]
+        pos += 4
+        r = r_min + (r_max - r_min) * u[0]
         if z_margin > 0:
-            c = rng.uniform((0.0, 0.0, z_margin + r), (nx, ny, nz - 1 - z_margin - r))
+            lo = (0.0, 0.0, z_margin + r)
+            hi = (float(nx), float(ny), nz - 1 - z_margin - r)
         else:
-            c = rng.uniform((0.0, 0.0, 0.0), (nx, ny, nz))
+            lo = (0.0, 0.0, 0.0)
+            hi = (float(nx), float(ny), float(nz))
+        lo_c[:] = lo
+        c = lo_c + (np.array(hi) - lo_c) * u[1:4]
         ci = (min(int(c[0] / cell), ncx - 1), min(int(c[1] / cell), ncy - 1),
               min(int(c[2] / cell), ncz - 1))
-        ok = True
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    for j in cells.get((ci[0] + dx, ci[1] + dy, ci[2] + dz), ()):
-                        d2 = float(np.sum((centres[j] - c) ** 2))
-                        if d2 < (radii[j] + r) ** 2:
-                            ok = False
-                            break
-                    if not ok:
-                        break
-                if not ok:
-                    break
-            if not ok:
-                break
-        if not ok:
-            continue
-        cells.setdefault(ci, []).append(len(centres))
-        centres.append(c)
-        radii.append(r)
+        nb = grid[ci[0]:ci[0] + 3, ci[1]:ci[1] + 3, ci[2]:ci[2] + 3].ravel()
+        nb = nb[nb >= 0]
+        if nb.size:
+            d2 = ((C[nb] - c) ** 2).sum(axis=1)
+            if np.any(d2 < (R[nb] + r) ** 2):
+                continue
+        if n_acc == cap:
+            cap *= 2
+            C = np.resize(C, (cap, 3))
+            R = np.resize(R, cap)
+        C[n_acc] = c
+        R[n_acc] = r
+        g = (ci[0] + 1, ci[1] + 1, ci[2] + 1)
+        if fill[g] >= K:
+            raise RuntimeError("rsa_spheres: cell list overflow")
+        grid[g + (fill[g],)] = n_acc
+        fill[g] += 1
+        n_acc += 1
         if solid_mask is None:
             solid += 4.0 / 3.0 * np.pi * r ** 3
         else:
@@ -160,6 +183,7 @@ def rsa_spheres(nx: int, ny: int, nz: int, r_min: float, r_max: float, porosity:
             sub = solid_mask[box]
             solid += float(np.count_nonzero(inside & ~sub))
             sub |= inside
+    centres, radii = C[:n_acc], R[:n_acc]
     return np.array(centres, dtype=np.float64).reshape(-1, 3), np.array(radii, dtype=np.float64)
 
 

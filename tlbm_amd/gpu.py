@@ -18,6 +18,7 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "tlbm_gpu.h")
 STORAGE = {"f64": 0, "f32": 1, "f16": 2, "f32x": 3}
 BYTES_PER_VALUE = {"f64": 8, "f32": 4, "f16": 2, "f32x": 4}
 LATTICE_Q = {"D3Q15": 15, "D3Q19": 19, "D3Q27": 27}
+ENONFINITE = -5  # TLBM_ENONFINITE: the state holds NaN or infinite populations
 
 
 class TlbmError(RuntimeError):
@@ -179,10 +180,17 @@ class Lattice:
         arrs = [_f64(v, self.N) for v in (ux, uy, uz, rho)]
         _check(load_library().tlbm_gpu_init_from_macros(self._h, *[a.ctypes.data for a in arrs]))
 
-    def reduce(self) -> dict:
+    def reduce(self, allow_nonfinite: bool = False) -> dict:
+        """FP64 mass, momentum and residual of the state.  A state holding
+        NaN or infinite populations raises TlbmError (code ENONFINITE)
+        unless allow_nonfinite, which returns the sums with "finite" False."""
+        L = load_library()
         out = np.zeros(5, np.float64)
-        _check(load_library().tlbm_gpu_reduce(self._h, out.ctypes.data))
-        return {"mass": out[0], "momentum": out[1:4].copy(), "residual": out[4]}
+        rc = L.tlbm_gpu_reduce(self._h, out.ctypes.data)
+        if not (rc == ENONFINITE and allow_nonfinite):
+            _check(rc)
+        return {"mass": out[0], "momentum": out[1:4].copy(), "residual": out[4],
+                "finite": rc == 0}
 
     def time_average(self, enable: bool = True) -> None:
         _check(load_library().tlbm_gpu_time_average(self._h, int(bool(enable))))

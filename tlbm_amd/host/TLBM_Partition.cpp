@@ -317,6 +317,15 @@ void TLBM_Partition::take_LBM_time_step(bool isEven) {
 
 void TLBM_Partition::synchronize() { check_rc(tlbm_gpu_sync(gpu_), "sync"); }
 
+void TLBM_Partition::check_finite(int ts) {
+    sync_populations_to_device();
+    double red[5];
+    const int rc = tlbm_gpu_reduce(gpu_, red);
+    if (rc == TLBM_ENONFINITE)
+        throw std::runtime_error("time step " + std::to_string(ts) + ": " + tlbm_gpu_last_error());
+    check_rc(rc, "state check");
+}
+
 template <class Real>
 void TLBM_Partition::write_field(const std::string &name, const std::vector<double> &v) {
     std::vector<Real> out(v.begin(), v.end());

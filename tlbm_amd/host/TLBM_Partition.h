@@ -73,6 +73,10 @@ public:
     int num_speeds() const { return Q_; }
     int storage() const { return storage_; }
     void synchronize();
+    // Divergence guard (no reference counterpart: its loop, src/tLBM.cpp:
+    // 45-62, steps on through NaN).  Throws std::runtime_error naming
+    // time step `ts` when any population of the state is NaN or infinite.
+    void check_finite(int ts);
 
 private:
     void load_ndType(std::vector<uint8_t> &nd);

@@ -4,6 +4,7 @@
 // dumps included) it prints the step-only rate.
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <exception>
 
 #include "TLBM_Partition.h"
@@ -18,13 +19,18 @@ int main() {
         const int warmupTs = myPart.get_warmupTs();
         const int plotFreq = myPart.get_plot_freq();
         const int timeAvgFlag = myPart.get_time_avg_flag();
+        // the divergence guard: every tsRepFreq steps the state is checked
+        // for NaN or infinite populations (TLBM_CHECK_FINITE=0: off)
+        const char *chk = std::getenv("TLBM_CHECK_FINITE");
+        const bool checkFinite = !(chk && chk[0] == '0' && chk[1] == '\0');
         using clock = std::chrono::steady_clock;
         const auto timeStart = clock::now();
         double dumpSeconds = 0.0;
         for (int ts = 0; ts < numTs; ++ts) {
-            if ((rank == 0) && ((ts + 1) % tsRepFreq == 0))
-                std::printf("Executing time step: %d \n", ts + 1);
+            const bool report = (ts + 1) % tsRepFreq == 0;
+            if ((rank == 0) && report) std::printf("Executing time step: %d \n", ts + 1);
             myPart.take_LBM_time_step(ts % 2);
+            if (report && checkFinite) myPart.check_finite(ts + 1);
             if (((ts + 1) % plotFreq == 0) && ((ts + 1) > warmupTs)) {
                 const auto d0 = clock::now();
                 myPart.write_data();
