@@ -1,6 +1,7 @@
 """The C-ABI library loads, exports what include/tlbm_gpu.h declares, and
 rejects bad input before touching a GPU (no compute calls here)."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -48,3 +49,29 @@ def test_create_rejects_type5_before_its_uy_sources():
 def test_create_rejects_short_node_array():
     with pytest.raises(ValueError):
         gpu.Lattice(19, 4, 4, 4, np.zeros(63, np.uint8), omega=1.2)
+
+
+def test_params_struct_layout_matches_header():
+    """The ctypes mirror of tlbm_params (tlbm_amd/gpu.py) has the size and
+    field offsets the C compiler gives the header's struct."""
+    import lattice_tables
+    fields = [f for f, _ in gpu.Params._fields_]
+    src = "#include <stdio.h>\n#include <stddef.h>\n#include \"tlbm_gpu.h\"\nint main(void) {\n"
+    src += '    printf("size %zu\\n", sizeof(tlbm_params));\n'
+    for f in fields:
+        src += f'    printf("{f} %zu %zu\\n", offsetof(tlbm_params, {f}), sizeof(((tlbm_params *)0)->{f}));\n'
+    src += "    return 0;\n}\n"
+    out = lattice_tables._run_c_probe(src, ["-I", os.path.join(lattice_tables.ROOT, "include")], [])
+    rows = dict((ln.split()[0], ln.split()[1:]) for ln in out.strip().splitlines())
+    assert int(rows["size"][0]) == ctypes.sizeof(gpu.Params)
+    for f in fields:
+        off, size = (int(v) for v in rows[f])
+        assert getattr(gpu.Params, f).offset == off, f
+        assert getattr(gpu.Params, f).size == size, f
+    # every field of the header's struct is mirrored
+    header = open(gpu.HEADER_PATH).read()
+    body = header[header.index("typedef struct tlbm_params {"):header.index("} tlbm_params;")]
+    import re
+    declared = re.findall(r"\b(?:int|double)\s+([a-z_, ]+);", body)
+    names = [n.strip() for group in declared for n in group.split(",")]
+    assert names == fields

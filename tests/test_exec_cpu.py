@@ -127,32 +127,48 @@ def test_ndtype_sidecar_is_read_and_validated(tmp_path):
     p = good_params()
     nd = nd_for(p).astype(int)
     d = case_dir(tmp_path, p, nd)
+    text = os.path.join(d, "ndType.lbm")
     bad = nd.copy()
     bad[7] = 4
-    io_formats.write_ndtype_bin(os.path.join(d, "ndType.bin"), bad)  # newer than ndType.lbm
+    io_formats.write_ndtype_bin(os.path.join(d, "ndType.bin"), bad, text_path=text)
     r = run_exec(d)
     assert r.returncode == 1 and "ndType.bin: node 7 has type 4" in r.stderr
-    io_formats.write_ndtype_bin(os.path.join(d, "ndType.bin"), nd[:-1])
+    io_formats.write_ndtype_bin(os.path.join(d, "ndType.bin"), nd[:-1], text_path=text)
     r = run_exec(d)
     assert r.returncode == 1 and "ndType.bin has" in r.stderr
     with open(os.path.join(d, "ndType.bin"), "wb") as fh:
-        fh.write(b"NOTASIDE" + bytes(8))
+        fh.write(b"NOTASIDE" + bytes(24))
     r = run_exec(d)
     assert r.returncode == 1 and "not an ndType sidecar" in r.stderr
+    # without a text file the sidecar is read as it is
+    os.remove(text)
+    io_formats.write_ndtype_bin(os.path.join(d, "ndType.bin"), bad)
+    r = run_exec(d)
+    assert r.returncode == 1 and "ndType.bin: node 7 has type 4" in r.stderr
 
 
-def test_stale_ndtype_sidecar_is_ignored(tmp_path):
+@pytest.mark.parametrize("touch", ["newer_sidecar", "older_sidecar"])
+def test_stale_ndtype_sidecar_is_ignored(tmp_path, touch):
+    """A sidecar written from another ndType.lbm is not used, whatever the
+    files' times: here the text file was rewritten (same node count and
+    length, one node changed) after the sidecar was made from the old one,
+    and the driver reads (and rejects) the new text."""
     p = good_params()
     nd = nd_for(p).astype(int)
     d = case_dir(tmp_path, p, nd)
-    io_formats.write_ndtype_bin(os.path.join(d, "ndType.bin"), nd[:-1])  # would be rejected
+    text = os.path.join(d, "ndType.lbm")
+    io_formats.write_ndtype_bin(os.path.join(d, "ndType.bin"), nd, text_path=text)
     bad = nd.copy()
-    bad[7] = 4
-    io_formats.write_int_lines(os.path.join(d, "ndType.lbm"), bad)
-    t = os.stat(os.path.join(d, "ndType.bin")).st_mtime
-    os.utime(os.path.join(d, "ndType.lbm"), (t + 10, t + 10))  # the text file is newer
+    bad[7] = 4 if nd[7] != 4 else 5
+    io_formats.write_int_lines(text, bad)
+    assert os.path.getsize(text) == int.from_bytes(
+        open(os.path.join(d, "ndType.bin"), "rb").read()[16:24], "little")
+    t = os.stat(text).st_mtime
+    side = t + 10 if touch == "newer_sidecar" else t - 10
+    os.utime(os.path.join(d, "ndType.bin"), (side, side))
     r = run_exec(d)
     assert r.returncode == 1 and "ndType.lbm: node 7 has type 4" in r.stderr
+    assert "ndType.bin was not written from this ndType.lbm" in r.stderr
 
 
 @pytest.fixture(scope="module")

@@ -113,10 +113,24 @@ def test_int_lines_match_printf_and_read_back(tmp_path):
 
 
 def test_ndtype_sidecar_round_trip(tmp_path):
+    """Header: magic, node count, then the size and CRC-32 of the text file
+    it was written from (zlib's CRC-32, the IEEE polynomial the driver
+    computes), zeros without one."""
+    import zlib
     nd = geometry.channel(7, 5, 9, obstacles=[geometry.sphere(3, 2, 4, 2)])
     path = os.path.join(str(tmp_path), "ndType.bin")
     io_formats.write_ndtype_bin(path, nd)
     assert np.array_equal(io_formats.read_ndtype_bin(path), nd)
     raw = open(path, "rb").read()
-    assert raw[:8] == b"TLBMNDT1" and int.from_bytes(raw[8:16], "little") == nd.size
-    assert len(raw) == 16 + nd.size
+    assert raw[:8] == b"TLBMNDT2" and int.from_bytes(raw[8:16], "little") == nd.size
+    assert raw[16:32] == bytes(16)
+    assert len(raw) == 32 + nd.size
+    text = os.path.join(str(tmp_path), "ndType.lbm")
+    io_formats.write_int_lines(text, nd)
+    io_formats.write_ndtype_bin(path, nd, text_path=text)
+    raw = open(path, "rb").read()
+    body = open(text, "rb").read()
+    assert int.from_bytes(raw[16:24], "little") == len(body)
+    assert int.from_bytes(raw[24:32], "little") == zlib.crc32(body)
+    # a known CRC-32 value: the standard check string
+    assert zlib.crc32(b"123456789") == 0xCBF43926

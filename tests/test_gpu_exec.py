@@ -160,24 +160,23 @@ def test_exec_restart_from_dump(gpu_lib, tmp_path):
 
 
 def test_exec_reads_the_binary_node_type_sidecar(gpu_lib, tmp_path):
-    """The configs' 24x24x96 D3Q15 case written with ndType.bin (tlbm_amd.cases)
-    and the text ndType.lbm replaced by garbage the driver would reject: the
-    run must use the sidecar and match the reference's 4-rank dumps."""
+    """The configs' 24x24x96 D3Q15 case written with ndType.bin (tlbm_amd.cases):
+    with the text ndType.lbm that it records, and with ndType.lbm removed,
+    the run uses the sidecar and matches the reference's 4-rank dumps."""
     from tlbm_amd import cases
     case = [c for c in goldens.load_cases() if c.name.startswith("channel_d3q15_24x24x96_f64")][0]
     d = str(tmp_path)
     p, nd = cases.write("channel", d, parts=4, binary=True, steps=case.params["numTs"],
                         plot_freq=case.params["numTs"])
-    bin_path = os.path.join(d, "ndType.bin")
-    t = os.stat(bin_path).st_mtime
-    with open(os.path.join(d, "ndType.lbm"), "w") as fh:
-        fh.write("7\n")                       # would fail if it were read
-    os.utime(os.path.join(d, "ndType.lbm"), (t - 10, t - 10))
-    r = run_exec(d, storage="f64")
-    assert r.returncode == 0, r.stderr
-    got = read_dump(d, 0, np.float64)
-    for f, ref in case.out.items():
-        assert rel_err(got[f], ref) <= TOL64, f
+    for text in ("kept", "removed"):
+        if text == "removed":
+            os.remove(os.path.join(d, "ndType.lbm"))
+        r = run_exec(d, storage="f64")
+        assert r.returncode == 0, r.stderr
+        assert "not written from this ndType.lbm" not in r.stderr
+        got = read_dump(d, 0, np.float64)
+        for f, ref in case.out.items():
+            assert rel_err(got[f], ref) <= TOL64, (text, f)
 
 
 def test_exec_stops_when_the_state_is_not_finite(gpu_lib, tmp_path):
@@ -203,3 +202,39 @@ def test_exec_stops_when_the_state_is_not_finite(gpu_lib, tmp_path):
         assert r.returncode == 1, (storage, r.stdout, r.stderr)
         assert "time step 10:" in r.stderr and "NaN or infinite" in r.stderr, storage
         assert "Test complete." not in r.stdout
+
+
+def test_exec_time_average_hdf5(gpu_lib, tmp_path):
+    """With TLBM_HDF5=1 the driver also writes timeAvg.h5 / timeAvg.xmf, the
+    files python/processTLBM.py:134-186 makes from the time-average sums:
+    each sum divided by Num_ts, velocity in physical units, pressure
+    relative to node pRefIdx, the speed, and no force group."""
+    nx, ny, nz = 8, 7, 12
+    nd = geometry.channel(nx, ny, nz, obstacles=[geometry.sphere(3, 3, 4, 2)])
+    p = io_formats.Params(latticeType="D3Q19", numTs=20, tsRepFreq=20, plotFreq=20, uLBM=0.05,
+                          omega=1.6, nx=nx, ny=ny, nz=nz, timeAvgFlag=1, tConvFact=0.002,
+                          lConvFact=0.01, pConvFact=3.5, pRefIdx=17, lx_p=0.7)
+    d = case_dir(tmp_path, p, nd)
+    r = run_exec(d, storage="f64", env={"TLBM_HDF5": "1"})
+    assert r.returncode == 0, r.stderr
+    h5 = os.path.join(d, "timeAvg.h5")
+    listing = subprocess.run(["/opt/conda/bin/h5ls", "-r", h5], capture_output=True, text=True).stdout
+    for ds in ("/velo_group/x_velo", "/velo_group/y_velo", "/velo_group/z_velo",
+               "/velo_group/velmag", "/pres_group/presmag"):
+        assert ds in listing, ds
+    assert "force_group" not in listing
+    assert "{12, 7, 8}" in listing
+    u_conv = 0.002 / 0.01
+    sums = {k: read_raw(d, f"{k}Avg.b_dat", np.float64) for k in ("u", "v", "w", "rho")}
+    u = sums["u"] / 20 / u_conv
+    v = sums["v"] / 20 / u_conv
+    w = sums["w"] / 20 / u_conv
+    for ds, want in (("/velo_group/x_velo", u), ("/velo_group/y_velo", v), ("/velo_group/z_velo", w)):
+        assert np.allclose(_h5(h5, ds, d), want, rtol=1e-6, atol=1e-9), ds
+    pres = sums["rho"] / 20 * 3.5
+    assert np.allclose(_h5(h5, "/pres_group/presmag", d), pres - pres[17], rtol=1e-5, atol=1e-6)
+    assert np.allclose(_h5(h5, "/velo_group/velmag", d), np.sqrt(u * u + v * v + w * w),
+                       rtol=1e-5, atol=1e-9)
+    xmf = open(os.path.join(d, "timeAvg.xmf")).read()
+    assert "timeAvg.h5:/velo_group/x_velo" in xmf and 'Dimensions="12 7 8"' in xmf
+    assert "force" not in xmf

@@ -83,7 +83,7 @@ def test_slab_runner_small(gpu_lib):
     rep = workloads.run_slab(gpus=8, steps=20, chunk=10, scale=16)
     assert rep["lattice"] == [64, 32, 4]
     for r in rep["runs"].values():
-        assert r["mlups"] > 0 and r["face_exchange_us"] > 0
+        assert r["mlups"] > 0 and r["face_exchange_us_modelled"] > 0
 
 
 

@@ -141,29 +141,68 @@ namespace {
 
 bool valid_type(long v) { return v == 0 || v == 1 || v == 2 || v == 3 || v == 5; }
 
-// ndType.bin: "TLBMNDT1", the node count (little-endian uint64), one byte
-// per node in global order.  Used in place of ndType.lbm when it exists and
-// is not older than the text file (a text file written after it wins).
-bool read_ndtype_sidecar(std::vector<uint8_t> &nd, size_t N) {
-    struct stat sb, st;
-    if (stat("ndType.bin", &sb) != 0) return false;
-    if (stat("ndType.lbm", &st) == 0) {
-        const bool older = sb.st_mtim.tv_sec < st.st_mtim.tv_sec ||
-                           (sb.st_mtim.tv_sec == st.st_mtim.tv_sec &&
-                            sb.st_mtim.tv_nsec < st.st_mtim.tv_nsec);
-        if (older) return false;
+// CRC-32 (IEEE 802.3, reflected, polynomial 0xEDB88320) of a whole file
+// and its size; false when the file cannot be opened.
+bool file_crc32(const char *path, uint64_t &size, uint32_t &crc) {
+    static uint32_t table[256];
+    static bool ready = false;
+    if (!ready) {
+        for (uint32_t i = 0; i < 256; ++i) {
+            uint32_t c = i;
+            for (int k = 0; k < 8; ++k) c = (c & 1u) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
+            table[i] = c;
+        }
+        ready = true;
     }
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return false;
+    std::vector<unsigned char> buf(size_t(1) << 22);
+    uint32_t c = 0xFFFFFFFFu;
+    size = 0;
+    size_t got;
+    while ((got = std::fread(buf.data(), 1, buf.size(), fp)) > 0) {
+        for (size_t i = 0; i < got; ++i) c = table[(c ^ buf[i]) & 0xFFu] ^ (c >> 8);
+        size += got;
+    }
+    std::fclose(fp);
+    crc = c ^ 0xFFFFFFFFu;
+    return true;
+}
+
+uint64_t le64(const unsigned char *b) {
+    uint64_t v = 0;
+    for (int k = 7; k >= 0; --k) v = (v << 8) | b[k];
+    return v;
+}
+
+// ndType.bin: "TLBMNDT2", then the node count, the size and the CRC-32 of
+// the ndType.lbm it was written from (zeros when none), each a
+// little-endian uint64, then one byte per node in global order.  Used in
+// place of ndType.lbm when that file is absent, or when it is the very file
+// recorded (same size and CRC-32); otherwise the text file is read.
+bool read_ndtype_sidecar(std::vector<uint8_t> &nd, size_t N) {
+    struct stat sb;
+    if (stat("ndType.bin", &sb) != 0) return false;
     FILE *fp = std::fopen("ndType.bin", "rb");
     if (!fp) throw std::runtime_error("Could not open ndType.bin");
-    char magic[8];
-    unsigned char cnt[8];
-    if (std::fread(magic, 1, 8, fp) != 8 || std::memcmp(magic, "TLBMNDT1", 8) != 0 ||
-        std::fread(cnt, 1, 8, fp) != 8) {
+    unsigned char head[32];
+    if (std::fread(head, 1, 32, fp) != 32 || std::memcmp(head, "TLBMNDT2", 8) != 0) {
         std::fclose(fp);
         throw std::runtime_error("ndType.bin: not an ndType sidecar");
     }
-    uint64_t n = 0;
-    for (int k = 7; k >= 0; --k) n = (n << 8) | cnt[k];
+    const uint64_t n = le64(head + 8), text_size = le64(head + 16), text_crc = le64(head + 24);
+    struct stat st;
+    if (stat("ndType.lbm", &st) == 0) {
+        uint64_t size = 0;
+        uint32_t crc = 0;
+        const bool same = uint64_t(st.st_size) == text_size && file_crc32("ndType.lbm", size, crc) &&
+                          size == text_size && uint64_t(crc) == text_crc;
+        if (!same) {
+            std::fclose(fp);
+            std::fprintf(stderr, "ndType.bin was not written from this ndType.lbm; reading ndType.lbm\n");
+            return false;
+        }
+    }
     if (n != N) {
         std::fclose(fp);
         throw std::runtime_error("ndType.bin has " + std::to_string(n) + " entries, expected " +
@@ -301,6 +340,27 @@ void TLBM_Partition::set_data_member(double val, int nd, int spd) {
     pop_dirty_ = true;
 }
 
+void TLBM_Partition::copy_populations(double *f) {
+    sync_populations_to_device();
+    check_rc(tlbm_gpu_get_populations(gpu_, f), "get populations");
+}
+
+void TLBM_Partition::load_populations(const double *f) {
+    check_rc(tlbm_gpu_set_populations(gpu_, f), "set populations");
+    pop_cached_ = false;
+    pop_dirty_ = false;
+}
+
+void TLBM_Partition::process_node_list(double *, double *, const std::set<int> &) {
+    throw std::logic_error(
+        "process_node_list: the GPU step updates every node at once; call take_LBM_time_step");
+}
+
+void TLBM_Partition::process_node_list(float *, float *, const std::set<int> &) {
+    throw std::logic_error(
+        "process_node_list: the GPU step updates every node at once; call take_LBM_time_step");
+}
+
 void TLBM_Partition::sync_populations_to_device() {
     if (pop_dirty_) {
         check_rc(tlbm_gpu_set_populations(gpu_, pop_cache_.data()), "set populations");
@@ -372,4 +432,6 @@ void TLBM_Partition::write_time_avg_data() {
         else
             write_field<float>(names[i], *fields[i]);
     }
+    if (env_flag("TLBM_HDF5"))
+        tlbm_host::write_time_avg_h5("timeAvg.h5", "timeAvg.xmf", thisProblem, u, v, w, r);
 }

@@ -13,11 +13,14 @@
 // (default f32, the reference's real = float); .b_dat dumps are written as
 // double for f64 and as float otherwise.  TLBM_EXACT_WEIGHTS=1 selects the
 // exact D3Q15/D3Q19 weights instead of the reference's float literals.
-// TLBM_HDF5=1 also writes out<k>.h5 / data<k>.xmf on every dump (the files
+// TLBM_HDF5=1 also writes out<k>.h5 / data<k>.xmf on every dump and
+// timeAvg.h5 / timeAvg.xmf with the time average (the files
 // python/processTLBM.py makes from the dumps), when built with HDF5.
+// TLBM_CHECK_FINITE=0 turns off the driver's divergence guard.
 #pragma once
 
 #include <cstdint>
+#include <set>
 #include <string>
 #include <vector>
 
@@ -54,11 +57,39 @@ public:
     int get_num_global_nodes();
     int is_restart();
     int get_time_avg_flag();
-    // Population spd of node nd that the next step reads (the reference's
-    // get_data_member / set_data_member on its fOut buffer, which its
-    // callers reach through private pointers).
+    // The reference's accessors of a node-major population array
+    // (include/TLBM_Partition.h:51-52, src/TLBM_Partition.cpp:860-870):
+    // f[getIDx(Q, nd, spd)].  Its own arrays are private and the state here
+    // lives in HBM, so a caller's array is filled by copy_populations and
+    // given back by load_populations.  Real is double or float (the
+    // reference's compile-time `real`).
+    template <class Real>
+    Real get_data_member(const Real *f, const int nd, const int spd) {
+        return f[getIDx(Q_, nd, spd)];
+    }
+    template <class Real>
+    void set_data_member(Real *f, const Real val, const int nd, const int spd) {
+        f[getIDx(Q_, nd, spd)] = val;
+    }
+    // include/TLBM_Partition.h:57-60: node-major index of a population.
+    static inline unsigned getIDx(int nSpd, int nIdx, int spd) {
+        return unsigned(nIdx) * unsigned(nSpd) + unsigned(spd);
+    }
+    // The populations the next step reads, node-major (N * Q values), and
+    // their replacement (the reference's fOut buffer).
+    void copy_populations(double *f);
+    void load_populations(const double *f);
+    // Population spd of node nd that the next step reads, one at a time
+    // (a convenience over the two calls above).
     double get_data_member(int nd, int spd);
     void set_data_member(double val, int nd, int spd);
+    // include/TLBM_Partition.h:56: the reference updates the nodes of a
+    // list (boundary nodes, then interior nodes) from fIn into fOut.  One
+    // kernel updates every node of the lattice per step here, and the
+    // buffers are in HBM, so a partial update has no counterpart: it throws
+    // std::logic_error.  take_LBM_time_step is the whole step.
+    void process_node_list(double *fOut, double *fIn, const std::set<int> &nodeList);
+    void process_node_list(float *fOut, float *fIn, const std::set<int> &nodeList);
     // One time step (TLBM_Partition.cpp:910-947).  isEven only names which
     // of the two buffers the reference reads; the GPU handle keeps its own
     // ping-pong.  Queued on the GPU: returns before the step completes.

@@ -22,6 +22,14 @@ void write_dump_h5(const std::string &h5name, const std::string &xmfname, const 
                    const std::vector<double> &fx, const std::vector<double> &fy,
                    const std::vector<double> &fz);
 
+// timeAvg.h5 / timeAvg.xmf from the time-average sums: each sum divided by
+// Num_ts, then velo_group/{x_velo,y_velo,z_velo,velmag} and
+// pres_group/presmag in physical units as for a dump, no forces
+// (processTLBM.py:134-186, hdf5Helper.py:33-56, 140-200).
+void write_time_avg_h5(const std::string &h5name, const std::string &xmfname, const Problem &p,
+                       const std::vector<double> &u_sum, const std::vector<double> &v_sum,
+                       const std::vector<double> &w_sum, const std::vector<double> &rho_sum);
+
 // /velocity/{x,y,z} and /density/rho, float32, global node order
 // (python/tLBM_create_restart.py:55-66), read as float32 and widened.
 void read_restart(const std::string &path, size_t n, std::vector<double> &ux,

@@ -13,6 +13,7 @@
 from __future__ import annotations
 
 import os
+import zlib
 from dataclasses import dataclass
 
 import numpy as np
@@ -135,25 +136,45 @@ def write_int_lines(path: str, values) -> None:
 # Binary sidecar of ndType.lbm: magic, node count (little-endian uint64),
 # then one byte per node in global order.  The drop-in driver reads it in
 # place of the text file when it is present and not older than ndType.lbm.
-NDTYPE_BIN_MAGIC = b"TLBMNDT1"
+NDTYPE_BIN_MAGIC = b"TLBMNDT2"
 
 
-def write_ndtype_bin(path: str, nd_type) -> None:
+def _text_signature(text_path):
+    """(size in bytes, CRC-32) of a text file, or (0, 0) when absent."""
+    if text_path is None or not os.path.exists(text_path):
+        return 0, 0
+    crc = 0
+    with open(text_path, "rb") as fh:
+        while True:
+            block = fh.read(1 << 22)
+            if not block:
+                break
+            crc = zlib.crc32(block, crc)
+    return os.path.getsize(text_path), crc
+
+
+def write_ndtype_bin(path: str, nd_type, text_path=None) -> None:
+    """The binary node-type sidecar: "TLBMNDT2", the node count, the size
+    and CRC-32 of the ndType.lbm it was made from (zeros when none; each
+    a little-endian uint64), then one byte per node in global order.  The
+    driver reads it in place of ndType.lbm only when that text file is
+    absent or is the one recorded."""
     nd = np.asarray(nd_type).ravel()
     if nd.size and (nd.min() < 0 or nd.max() > 255):
         raise ValueError("node types must fit in a byte")
+    size, crc = _text_signature(text_path)
     with open(path, "wb") as fh:
         fh.write(NDTYPE_BIN_MAGIC)
-        fh.write(np.uint64(nd.size).tobytes())
+        fh.write(np.array([nd.size, size, crc], np.uint64).astype("<u8").tobytes())
         fh.write(nd.astype(np.uint8).tobytes())
 
 
 def read_ndtype_bin(path: str) -> np.ndarray:
     with open(path, "rb") as fh:
-        head = fh.read(16)
-        if len(head) != 16 or head[:8] != NDTYPE_BIN_MAGIC:
+        head = fh.read(32)
+        if len(head) != 32 or head[:8] != NDTYPE_BIN_MAGIC:
             raise ValueError(f"{path}: not an ndType sidecar")
-        n = int(np.frombuffer(head[8:], np.uint64)[0])
+        n = int(np.frombuffer(head[8:16], "<u8")[0])
         nd = np.frombuffer(fh.read(), np.uint8)
     if nd.size != n:
         raise ValueError(f"{path}: {nd.size} node types, header says {n}")
@@ -180,7 +201,8 @@ def write_case(directory: str, p: Params, nd_type, parts=None, binary: bool = Fa
         parts = np.zeros(p.num_nodes, dtype=np.int64)
     write_int_lines(os.path.join(directory, "parts.lbm"), parts)
     if binary:
-        write_ndtype_bin(os.path.join(directory, "ndType.bin"), nd_type)
+        write_ndtype_bin(os.path.join(directory, "ndType.bin"), nd_type,
+                         text_path=os.path.join(directory, "ndType.lbm"))
 
 
 def read_bdat(directory: str, field: str, k: int, dtype) -> np.ndarray:

@@ -31,7 +31,8 @@
   body's mid-section, no inlet or outlet), periodic in z -- a ring of one,
   whose own z wrap stands in for the ghost planes -- timed for FP16 +
   network and FP32 Smagorinsky beside the face exchange a sharded step would
-  overlap (the 5 crossing D3Q19 directions of one face over one xGMI link).
+  overlap, modelled (not measured) as the bytes of the 5 crossing D3Q19
+  directions of one face over one xGMI link's 153 GB/s.
 
 Every run is timed with HIP events around batches of steps (kernel time on
 the lattice's stream), so MLUPS = nodes x steps / time, counting every node
@@ -363,9 +364,10 @@ def run_slab(gpus: int = 8, steps: int = 1000, chunk: int = 500, scale: int = 1,
     its real shape.  The slab is periodic in z (a ring of one: the step's
     own z wrap fills the ghost planes locally), which is the work each GPU
     does per step.  It is timed for FP16 + network and FP32 Smagorinsky and
-    set beside the face exchange the sharded step would have to hide: the
-    populations crossing a face (the 5 D3Q19 directions with e_z = +1, or
-    -1) of nx ny nodes over one xGMI link at `link_gbs`."""
+    set beside the face exchange the sharded step would have to hide,
+    modelled as the populations crossing a face (the 5 D3Q19 directions with
+    e_z = +1, or -1) of nx ny nodes over one xGMI link at `link_gbs` (no
+    second GPU runs here)."""
     from .closure import load_network, split_closure
     dims, nd, z0 = slab_case(gpus, scale)
     nx, ny, nzs = dims
@@ -387,7 +389,8 @@ def run_slab(gpus: int = 8, steps: int = 1000, chunk: int = 500, scale: int = 1,
         r = _rate(nd.size, 19, storage, steps, ms)
         face = 5 * nx * ny * STORAGE_BYTES[storage]
         r["face_bytes"] = face
-        r["face_exchange_us"] = face / (link_gbs * 1e3)
+        # modelled, not measured: the face's bytes over one link's bandwidth
+        r["face_exchange_us_modelled"] = face / (link_gbs * 1e3)
         # the step's interior planes: the window the face exchange overlaps with
         r["interior_us"] = r["ms_per_step"] * 1e3 * (nzs - 2) / nzs
         out["runs"][f"{storage}_{'network' if cl is not None else 'smagorinsky'}"] = r
