@@ -45,8 +45,13 @@ learned replacement and compares it with that form:
    long run from uniform flow, per window -- plus the per-step cost of
    both.
 
-A closure is None (Smagorinsky) or (params, feature_scale, output_scale[,
-tau_offset]).
+Smagorinsky is calibrated the same way, on the same training window, for a
+like-for-like comparison: a grid over its constant cs and the same base
+offsets c.  The held-out evaluation scores the network against both the
+plain (cs of the fine run, no offset) and the calibrated Smagorinsky.
+
+A closure is None (Smagorinsky at the fine run's cs), a Smagorinsky(cs,
+tau_offset), or (params, feature_scale, output_scale[, tau_offset]).
 
 Everything runs through the C-ABI (tlbm_amd.gpu); numpy only forms the
 coarse-grid targets and features.
@@ -410,11 +415,31 @@ def train_closure(case: ClosureCase, data: dict, *, steps=(2000, 1500, 1500), ba
     return params, cl_in, cl_out, report
 
 
+@dataclasses.dataclass(frozen=True)
+class Smagorinsky:
+    """The reference's closure (include/LatticeStructure.hpp:140-153) with
+    its own constant and a base relaxation-time offset: the coarse lattice
+    runs at tau0 + tau_offset with Smagorinsky cs on top."""
+    cs: float
+    tau_offset: float = 0.0
+
+
 def split_closure(closure):
     """(network (params, feature_scale, output_scale) or None, tau offset)."""
     if closure is None:
         return None, 0.0
+    if isinstance(closure, Smagorinsky):
+        return None, float(closure.tau_offset)
     return tuple(closure[:3]), (float(closure[3]) if len(closure) > 3 else 0.0)
+
+
+def closure_cs(case: ClosureCase, closure) -> float:
+    """Smagorinsky constant of a closure's run (0 under a network)."""
+    if closure is None:
+        return case.cs
+    if isinstance(closure, Smagorinsky):
+        return float(closure.cs)
+    return 0.0
 
 
 class CoarseRun:
@@ -430,7 +455,7 @@ class CoarseRun:
         net, offset = split_closure(closure)
         self.lat = Lattice(19, cx, cy, cz, nd_coarse, storage=storage,
                            omega=1.0 / (case.tau_coarse + offset), u_lbm=case.u_in,
-                           cs=case.cs if net is None else 0.0)
+                           cs=closure_cs(case, closure))
         self.lat.set_inlet_profile(prof)
         if net is not None:
             self.lat.set_closure(*net)
@@ -557,20 +582,9 @@ def calibrate(case: ClosureCase, nd_fine, data, closure, multipliers=(0.25, 0.5,
     L2 (mean over horizons, runs started from training snapshots) over
     Smagorinsky's + training-window mean-velocity L2 over Smagorinsky's;
     runs that diverge are out.  Returns ((m, c), table)."""
-    nd_c = coarse_geometry(nd_fine, case)
-    u_f = np.stack(data["u"]).astype(np.float64)
-    ntr = case.snapshots - case.holdout
-    js = [int(j) for j in np.linspace(0, ntr - 1 - max(horizons), starts).astype(int)]
+    score, ref, js = _training_scorer(case, nd_fine, data, starts, horizons, storage)
     net, _ = split_closure(closure)
     params, cl_in, cl_out = net
-
-    def score(cl):
-        sh = short_horizon(case, nd_c, data, cl, js, horizons=horizons, storage=storage,
-                           u_f=u_f)
-        st = statistics(case, nd_c, data, cl, storage=storage, u_f=u_f)["train"]
-        return _finite(float(np.mean(list(sh.values())))), _finite(st["mean_l2"])
-
-    ref = score(None)
     table = {"smagorinsky": {"short": ref[0], "mean": ref[1]}}
     best, best_score = None, float("inf")
     for m in multipliers:
@@ -585,6 +599,51 @@ def calibrate(case: ClosureCase, nd_fine, data, closure, multipliers=(0.25, 0.5,
         raise RuntimeError("every calibrated network run diverged")
     _log(f"calibration on training window (starts {js}): smagorinsky short {ref[0]:.4g} "
          f"mean {ref[1]:.4g} -> m {best[0]} c {best[1]} (score {best_score:.4g})")
+    return best, table
+
+
+def _training_scorer(case: ClosureCase, nd_fine, data, starts, horizons, storage):
+    """The calibration score's parts on the TRAINING window -- mean
+    short-horizon L2 (runs started from `starts` training snapshots) and
+    the training-window mean-velocity L2 of a long run -- as a function of
+    the closure, with the plain Smagorinsky run's values and the starts."""
+    nd_c = coarse_geometry(nd_fine, case)
+    u_f = np.stack(data["u"]).astype(np.float64)
+    ntr = case.snapshots - case.holdout
+    js = [int(j) for j in np.linspace(0, ntr - 1 - max(horizons), starts).astype(int)]
+
+    def score(cl):
+        sh = short_horizon(case, nd_c, data, cl, js, horizons=horizons, storage=storage,
+                           u_f=u_f)
+        st = statistics(case, nd_c, data, cl, storage=storage, u_f=u_f)["train"]
+        return _finite(float(np.mean(list(sh.values())))), _finite(st["mean_l2"])
+
+    return score, score(None), js
+
+
+def calibrate_smagorinsky(case: ClosureCase, nd_fine, data, cs_values=(0.025, 0.05, 0.1),
+                          offsets=(0.0, 0.004, 0.008, 0.012), starts=4, horizons=(1, 4, 8),
+                          storage="f32"):
+    """The network's calibration applied to Smagorinsky: a grid over its
+    constant cs and the same base offsets c, the same training-window score
+    (each part relative to the plain Smagorinsky run's).  Returns
+    (Smagorinsky(cs, c), table)."""
+    score, ref, js = _training_scorer(case, nd_fine, data, starts, horizons, storage)
+    table = {"smagorinsky": {"short": ref[0], "mean": ref[1]}}
+    best, best_score = None, float("inf")
+    for cs in cs_values:
+        for c in offsets:
+            sh, mean = score(Smagorinsky(cs, c))
+            sc = sh / ref[0] + mean / ref[1]
+            table[f"cs{cs}/{c}"] = {"short": sh, "mean": mean, "score": sc}
+            _log(f"smagorinsky calibration cs {cs} c {c}: short {sh:.4g} mean {mean:.4g} "
+                 f"score {sc:.4g}")
+            if sc < best_score:
+                best, best_score = Smagorinsky(cs, c), sc
+    if best is None:
+        raise RuntimeError("every calibrated Smagorinsky run diverged")
+    _log(f"smagorinsky calibration (starts {js}) -> cs {best.cs} c {best.tau_offset} "
+         f"(score {best_score:.4g})")
     return best, table
 
 
@@ -642,8 +701,11 @@ def run_pipeline(case: ClosureCase, *, train_steps=(2000, 1500, 1500), cost_shap
     params, cl_in, cl_out, rep = train_closure(case, data, steps=train_steps)
     (m, c), table = calibrate(case, nd, data, (params, cl_in, cl_out), storage=storage)
     rep["calibration"] = {"multiplier": m, "tau_offset": c, "table": table}
+    smag, stable = calibrate_smagorinsky(case, nd, data, storage=storage)
+    rep["smagorinsky_calibration"] = {"cs": smag.cs, "tau_offset": smag.tau_offset,
+                                      "table": stable}
     closure = (params, cl_in, cl_out * m, c)
-    closures = {"smagorinsky": None, "network": closure,
+    closures = {"smagorinsky": None, "smagorinsky_calibrated": smag, "network": closure,
                 "network_uncalibrated": (params, cl_in, cl_out)}
     ev = evaluate(case, nd, data, closures, storage=storage)
     cost = {}
@@ -708,6 +770,7 @@ def main(argv=None) -> int:
     summary = {k: rep[k] for k in ("coarse", "evaluation", "cost", "seconds")}
     summary["apriori"] = rep["training"]["apriori"]
     summary["calibration"] = rep["training"]["calibration"]
+    summary["smagorinsky_calibration"] = rep["training"]["smagorinsky_calibration"]
     print(json.dumps(summary, indent=1))
     return 0
 
