@@ -59,6 +59,7 @@ struct StepArgs {
     UDiv div_nx, div_nxny;  // g / nxny and r / nx without an integer divide
     uint32_t kb;       // chunks a pull source lies at most before its node's chunk
     int rel;           // 1: PullRel offsets fit 32 bits on this lattice
+    int row_chunks;    // 1: rel, and every row lies in one chunk (CH a multiple of nx): PullRow
     uint32_t step;     // steps taken since creation: the stream of FP16 stochastic rounding
     uint64_t fbytes;   // bytes of one state buffer
     int dynamics;      // 1: BGK relaxation, other values: none (TLBM_Partition.cpp:795-807)
@@ -365,6 +366,50 @@ struct PullRel {
         const uint32_t rb = xs[ex + 1] + ys[ey + 1] + zs[ez + 1];
         const uint32_t off = rb + __umul24(rb >> (CB + kLog), (L::Q - 1) * CH * kSize);
         return RawBuf<st>::template load<NT ? kAuxNT : 0>(rsrc, off, uint32_t(s) * CH * kSize);
+    }
+};
+
+// PullRel on a lattice whose rows never straddle a chunk (CH a multiple of
+// nx): the chunk of a source node is the chunk of its row's first node, so
+// the chunk-crossing term of the offset, (r >> CB) (Q - 1) CH, is formed
+// once per source row -- the 9 combinations of e_y and e_z -- and each
+// population adds only its source's x offset to its row's offset: one
+// VALU add per population instead of an add, a shift and a multiply-add.
+template <class L, int ST>
+struct PullRow {
+    using st = typename StorageT<ST>::st;
+    static constexpr uint32_t kSize = sizeof(st);
+    static constexpr int kLog = kSize == 8 ? 3 : (kSize == 4 ? 2 : 1);
+    static constexpr int CB = StorageT<ST>::kCB;
+    static constexpr uint32_t CH = 1u << CB;
+    static_assert((uint64_t(L::Q - 1) * CH * kSize) < (1u << 24), "24-bit multiplier");
+    __amdgpu_buffer_rsrc_t rsrc;  // based at chunk c0 - kb, as PullRel's
+    uint32_t xs[3];               // byte offset of the source's x (e_x = -1, 0, +1)
+    uint32_t row[3][3];           // [e_y + 1][e_z + 1]: byte offset of the source row
+    __device__ inline void init(const st *fin, uint32_t g0, uint32_t x, uint32_t y,
+                                uint32_t zplane, const StepArgs &a) {
+        const uint32_t c0 = g0 >> CB;
+        const intptr_t back = (intptr_t(c0) - intptr_t(a.kb)) * intptr_t(L::Q * CH * kSize);
+        rsrc = make_rsrc(uintptr_t(fin) + back, uint64_t(intptr_t(a.fbytes) - back));
+        xs[0] = ((x + 1 == a.nx) ? 0 : x + 1) << kLog;
+        xs[1] = x << kLog;
+        xs[2] = ((x == 0) ? a.nx - 1 : x - 1) << kLog;
+        const uint32_t ys[3] = {(((y + 1 == a.ny) ? 0 : y + 1) * a.nx) << kLog, (y * a.nx) << kLog,
+                                (((y == 0) ? a.ny - 1 : y - 1) * a.nx) << kLog};
+        const uint32_t zr = (zplane - (c0 - a.kb) * CH) << kLog;  // modulo 2^32; the sums are exact
+        const uint32_t zs[3] = {zr + (a.nxny << kLog), zr, zr - (a.nxny << kLog)};
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t rb = ys[j] + zs[k];
+                row[j][k] = rb + __umul24(rb >> (CB + kLog), (L::Q - 1) * CH * kSize);
+            }
+    }
+    template <bool NT>
+    __device__ inline st load(int s, int ex, int ey, int ez) const {
+        return RawBuf<st>::template load<NT ? kAuxNT : 0>(rsrc, row[ey + 1][ez + 1] + xs[ex + 1],
+                                                          uint32_t(s) * CH * kSize);
     }
 };
 
@@ -711,7 +756,12 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
     const uint32_t y = udiv(r, a.div_nx);
     const uint32_t x = r - y * a.nx;
     T f[Q];
-    if (MODE == MODE_STEP && pull_rel_ok<BS>(blk * BS, a)) {
+    if (MODE == MODE_STEP && a.row_chunks && pull_rel_ok<BS>(blk * BS, a)) {
+        PullRow<L, ST> pr;
+        pr.init(fin, blk * BS, x, y, g - r, a);
+#pragma unroll
+        for (int s = 0; s < Q; ++s) f[s] = S::in(pr.template load<NTLOAD>(s, L::ex[s], L::ey[s], L::ez[s]), s, a);
+    } else if (MODE == MODE_STEP && pull_rel_ok<BS>(blk * BS, a)) {
         PullRel<L, ST> pr;
         pr.init(fin, blk * BS, x, y, g - r, a);
 #pragma unroll
@@ -807,7 +857,12 @@ k_lbm_closure(const typename StorageT<ST>::st *__restrict__ fin, typename Storag
     const uint32_t y = udiv(r, a.div_nx);
     const uint32_t x = r - y * a.nx;
     T f[Q];
-    if (pull_rel_ok<BS>(blockIdx.x * BS, a)) {
+    if (a.row_chunks && pull_rel_ok<BS>(blockIdx.x * BS, a)) {
+        PullRow<L, ST> pr;
+        pr.init(fin, blockIdx.x * BS, x, y, g - r, a);
+#pragma unroll
+        for (int s = 0; s < Q; ++s) f[s] = S::in(pr.template load<NTLOAD>(s, L::ex[s], L::ey[s], L::ez[s]), s, a);
+    } else if (pull_rel_ok<BS>(blockIdx.x * BS, a)) {
         PullRel<L, ST> pr;
         pr.init(fin, blockIdx.x * BS, x, y, g - r, a);
 #pragma unroll

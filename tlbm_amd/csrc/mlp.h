@@ -41,14 +41,20 @@ constexpr int kMlpIn = 7;
 constexpr int kMlpH = 32;
 constexpr int kMlpParams = kMlpH * kMlpIn + kMlpH + kMlpH * kMlpH + kMlpH + kMlpH + 1;
 
-// Weights laid out per lane for the MFMA operands (built by mlp_pack).
-struct MlpPacked {
+// Weights laid out per lane for the MFMA operands (built by mlp_pack).  The
+// values a lane reads from LDS sit at small constant offsets from one base
+// per lane half (hb[h]) or per lane (a1[l], a2[kb][l]), so the reads take
+// their offsets as immediates instead of an address computation each.
+struct MlpHalf {
+    float b1[16];  // b1 of unit 8q+4h+r at [4q+r]
+    float b2[16];
+    float w3[16];  // w3 rounded to bf16
+};
+struct alignas(16) MlpPacked {
+    MlpHalf hb[2];
+    float b3, pad[3];
     uint32_t a1[64][2];     // W1[l%32][4(l/32)+t], t = 0..3, bf16 pairs
     uint32_t a2[2][64][4];  // W2[l%32][u], u = 16kb + 8(t/4) + 4(l/32) + t%4, t = 0..7
-    float b1[2][16];        // b1 of unit 8q+4h+r at [h][4q+r]
-    float b2[2][16];
-    float w3[2][16];        // w3 rounded to bf16
-    float b3;
 };
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
@@ -94,11 +100,12 @@ inline void mlp_pack(const float *p, MlpPacked *m) {
         for (int q = 0; q < 4; ++q)
             for (int r = 0; r < 4; ++r) {
                 const int u = 8 * q + 4 * h + r;
-                m->b1[h][4 * q + r] = b1[u];
-                m->b2[h][4 * q + r] = b2[u];
-                m->w3[h][4 * q + r] = bf16_to_f32(f32_to_bf16(w3[u]));
+                m->hb[h].b1[4 * q + r] = b1[u];
+                m->hb[h].b2[4 * q + r] = b2[u];
+                m->hb[h].w3[4 * q + r] = bf16_to_f32(f32_to_bf16(w3[u]));
             }
     m->b3 = b3[0];
+    m->pad[0] = m->pad[1] = m->pad[2] = 0.f;
 }
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -178,8 +185,9 @@ __device__ inline float mlp_wave(const float (&x)[kMlpIn], const MlpPacked *__re
     float part[2];
 #pragma unroll
     for (int blk = 0; blk < 2; ++blk) {
-        const f32x16 c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, xb[blk], load16(m->b1[h]), 0, 0, 0);
-        f32x16 d = load16(m->b2[h]);
+        const MlpHalf &H = m->hb[h];
+        const f32x16 c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, xb[blk], load16(H.b1), 0, 0, 0);
+        f32x16 d = load16(H.b2);
 #pragma unroll
         for (int kb = 0; kb < 2; ++kb) {
             // registers 8kb .. 8kb+7: the 8 items of K block kb in this lane
@@ -189,7 +197,7 @@ __device__ inline float mlp_wave(const float (&x)[kMlpIn], const MlpPacked *__re
         }
         float p = 0.f;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) p += m->w3[h][r] * relu(d[r]);
+        for (int r = 0; r < 16; ++r) p += H.w3[r] * relu(d[r]);
         part[blk] = p;
     }
     const float pa = part[0] + __shfl_xor(part[0], 32);
