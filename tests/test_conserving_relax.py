@@ -169,3 +169,21 @@ def test_fp16_equilibrium_correction(case):
             mom = sum(plain[s] * int(e[s, a]) for s in range(q))
             assert mom == k2f * rho * u[a]
             assert abs(mom - k2 * rho * u[a]) > Fr(5, 10 ** 9) * abs(rho * u[a])
+
+
+@pytest.mark.parametrize("q", [15, 19, 27])
+def test_reconstruction_partners_share_their_weight(q):
+    """FP16 steps in the scaled deviation h = (f - w)/w.  The bounce-back
+    (f_s <- f_bb(s)) and the inlet/outlet reconstruction
+    f_unk = feq_unk + f_par - feq_par keep their form in that scale only
+    because a direction and its partner have the same weight: checked for
+    every pair of the tables (reference and exact weights)."""
+    e, bb, w, w_exact = lattice_tables.tables(q)
+    _, _, _, _, unk, _ = lattice_tables._parse_tables(lattice_tables._tables(q))
+    for weights in (w, w_exact, [float(np.float32(x)) for x in w],
+                    [float(np.float32(x)) for x in w_exact]):
+        for s in range(q):
+            assert weights[s] == weights[bb[s]], (q, s)
+        for in_unk, in_par, out_unk, out_par, _, _ in unk:
+            assert weights[in_unk] == weights[in_par], (q, in_unk, in_par)
+            assert weights[out_unk] == weights[out_par], (q, out_unk, out_par)

@@ -446,12 +446,14 @@ class CoarseRun:
     """A coarse lattice with either closure; a network closure's tau offset
     shifts the lattice's base relaxation time."""
 
-    def __init__(self, case: ClosureCase, nd_coarse, closure, storage="f32"):
+    def __init__(self, case: ClosureCase, nd_coarse, closure, storage="f32", inlet_scale=None):
         cx, cy, cz = case.coarse
         self.case = case
         self.nd = nd_coarse
         self.fluid = nd_coarse != geometry.SOLID
         _, prof = inlet_profiles(case)
+        if inlet_scale is not None:
+            prof = prof * inlet_scale
         net, offset = split_closure(closure)
         self.lat = Lattice(19, cx, cy, cz, nd_coarse, storage=storage,
                            omega=1.0 / (case.tau_coarse + offset), u_lbm=case.u_in,
@@ -523,13 +525,16 @@ def short_horizon(case: ClosureCase, nd_c, data, closure, starts, horizons=(1, 4
     return {str(h): float(np.mean(errs[h])) for h in horizons}
 
 
-def statistics(case: ClosureCase, nd_c, data, closure, storage="f32", u_f=None):
+def statistics(case: ClosureCase, nd_c, data, closure, storage="f32", u_f=None,
+               perturb: float = 0.0):
     """One coarse run from uniform flow over the fine run's time window
     (acoustic scaling), sampled at the snapshot times.  Per window --
     "train" (the training snapshots) and "heldout" (the rest) -- the
     relative L2 of the mean velocity and of the rms of the vortical
     fluctuation (deplane) against the filtered fine run over the same
-    snapshots."""
+    snapshots.  perturb > 0 multiplies the coarse inlet profile by
+    1 + perturb U[-1, 1] (seeded): how far the flow itself moves these
+    statistics for a change that small."""
     fluid = nd_c == geometry.FLUID
     if u_f is None:
         u_f = np.stack(data["u"]).astype(np.float64)
@@ -537,7 +542,11 @@ def statistics(case: ClosureCase, nd_c, data, closure, storage="f32", u_f=None):
     windows = {"train": (0, ntr), "heldout": (ntr, case.snapshots)}
     half = case.every // 2
     acc = {w: [0.0, 0.0, 0.0] for w in windows}
-    with CoarseRun(case, nd_c, closure, storage) as run:
+    scale = None
+    if perturb > 0:
+        cx, cy, _ = case.coarse
+        scale = 1.0 + perturb * np.random.default_rng(11).uniform(-1.0, 1.0, cx * cy)
+    with CoarseRun(case, nd_c, closure, storage, inlet_scale=scale) as run:
         run.lat.init_from_macros(*uniform_start(nd_c, case.u_in))
         run.lat.step(case.spinup // 2)
         for k in range(case.snapshots):
@@ -667,6 +676,10 @@ def evaluate(case: ClosureCase, nd_fine, data, closures: dict, *, starts=4,
                                             storage=storage, u_f=u_f),
                      "stats": statistics(case, nd_c, data, cl, storage=storage, u_f=u_f)}
         _log(f"{name}: {out[name]}")
+    # the flow's own sensitivity: Smagorinsky with its inlet perturbed by 1e-7
+    out["smagorinsky_inlet_perturbed_1e-7"] = {
+        "stats": statistics(case, nd_c, data, None, storage=storage, u_f=u_f, perturb=1e-7)}
+    _log(f"smagorinsky, inlet perturbed by 1e-7: {out['smagorinsky_inlet_perturbed_1e-7']}")
     return out
 
 

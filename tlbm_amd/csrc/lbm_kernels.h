@@ -112,6 +112,7 @@ template <> struct StorageT<ST_F64> {
     static constexpr int kCB = 12;  // 32 KiB chunks
     using T = double;
     static constexpr bool kShift = false;
+    static constexpr bool kScaled = false;
     __device__ static inline T in(st r, int, const StepArgs &) { return r; }
     __device__ static inline st out(T v, int, const StepArgs &) { return v; }
     __device__ static inline T load(const st *p, int, const StepArgs &) { return *p; }
@@ -128,6 +129,7 @@ template <> struct StorageT<ST_F32> {
     static constexpr int kCB = 13;  // 32 KiB chunks
     using T = float;
     static constexpr bool kShift = true;
+    static constexpr bool kScaled = false;
     // the weight the stored deviation is taken from
     __device__ static inline double shift(int s, const StepArgs &a) { return double(a.wf[s]); }
     __device__ static inline T in(st r, int, const StepArgs &) { return r; }
@@ -153,6 +155,7 @@ template <> struct StorageT<ST_F32X> {
     static constexpr int kCB = 13;  // 32 KiB chunks
     using T = double;
     static constexpr bool kShift = true;
+    static constexpr bool kScaled = false;
     __device__ static inline double shift(int s, const StepArgs &a) { return a.w[s]; }
     __device__ static inline T in(st r, int, const StepArgs &) { return T(r); }
     __device__ static inline st out(T v, int, const StepArgs &) { return st(v); }
@@ -174,11 +177,12 @@ template <> struct StorageT<ST_F16> {
     static constexpr int kCB = 14;  // 32 KiB chunks
     using T = float;
     static constexpr bool kShift = true;
+    // The time step computes in the stored scale itself, on h = d / w (see
+    // node_prepare): the gathered value is widened as it is, and the step's
+    // store rounds h without a rescale.
+    static constexpr bool kScaled = true;
     __device__ static inline double shift(int s, const StepArgs &a) { return double(a.wf[s]); }
-    // one mixed-precision FMA (f16 operand, f32 scale)
-    __device__ static inline T in(st r, int s, const StepArgs &a) {
-        return __builtin_fmaf(float(r), a.h_unscale[s], 0.0f);
-    }
+    __device__ static inline T in(st r, int, const StepArgs &) { return float(r); }
     __device__ static inline st out(T v, int s, const StepArgs &a) { return _Float16(v * a.h_scale[s]); }
     __device__ static inline T load(const st *p, int s, const StepArgs &a) {
         return float(*p) * a.h_unscale[s];
@@ -191,10 +195,10 @@ template <> struct StorageT<ST_F16> {
     // the flow is steady, and that bias accumulates (D3Q19 cylinder array:
     // velocity L2 against FP64 1.0e-2 after 20000 steps, against 8.0e-4
     // rounded stochastically).
-    __device__ static inline st step_out(T v, int s, const StepArgs &a, uint32_t r) {
+    __device__ static inline st step_out(T v, int, const StepArgs &, uint32_t r) {
         typedef __fp16 h2 __attribute__((ext_vector_type(2)));
         const h2 z = {0, 0};
-        const h2 x = __builtin_amdgcn_cvt_sr_f16_f32(z, v * a.h_scale[s], r, false);
+        const h2 x = __builtin_amdgcn_cvt_sr_f16_f32(z, v, r, false);
         return __builtin_bit_cast(_Float16, x[0]);
     }
     __device__ static inline T weight(int s, const StepArgs &a) { return a.wf[s]; }
@@ -460,21 +464,35 @@ __device__ inline T edot(int ex, int ey, int ez, T ux, T uy, T uz) {
 // step.  With COLLIDE it also applies the type-5 rule and the inlet/outlet
 // reconstruction, fills feq and returns true for nodes that relax; a solid
 // node is bounced back in place (feq = f) and returns false.
-template <class L, class T, bool SHIFT, bool COLLIDE>
+// SC (scaled, with SHIFT): f[] and feq[] hold h = d / w instead of d.  Every
+// rule is the same rewritten in that scale: sums of populations weigh each
+// h by its w (one FMA per term), the equilibrium loses its factor w, the
+// type-5 increment its w, and the bounce-back and inlet/outlet
+// reconstructions keep their form because a direction and its partner have
+// the same weight.
+template <class L, class T, bool SHIFT, bool COLLIDE, bool SC = false>
 __device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho, T &ux, T &uy,
                                     T &uz, const StepArgs &a, const T *__restrict__ uy_head,
                                     const T (&w)[L::Q], uint32_t g) {
     constexpr int Q = L::Q;
+    static_assert(!SC || SHIFT, "the scaled form is a deviation form");
     T dlt = T(0);  // rho - 1 (SHIFT only)
     // compute_macroscopic_data, LatticeStructure.hpp:69-90
     {
         T r = T(-0.0), mx = T(-0.0), my = T(-0.0), mz = T(-0.0);
 #pragma unroll
         for (int s = 0; s < Q; ++s) {
-            r += f[s];
-            if (L::ex[s] > 0) mx += f[s]; else if (L::ex[s] < 0) mx -= f[s];
-            if (L::ey[s] > 0) my += f[s]; else if (L::ey[s] < 0) my -= f[s];
-            if (L::ez[s] > 0) mz += f[s]; else if (L::ez[s] < 0) mz -= f[s];
+            if constexpr (SC) {
+                r = fma_t(w[s], f[s], r);
+                if (L::ex[s] > 0) mx = fma_t(w[s], f[s], mx); else if (L::ex[s] < 0) mx = fma_t(-w[s], f[s], mx);
+                if (L::ey[s] > 0) my = fma_t(w[s], f[s], my); else if (L::ey[s] < 0) my = fma_t(-w[s], f[s], my);
+                if (L::ez[s] > 0) mz = fma_t(w[s], f[s], mz); else if (L::ez[s] < 0) mz = fma_t(-w[s], f[s], mz);
+            } else {
+                r += f[s];
+                if (L::ex[s] > 0) mx += f[s]; else if (L::ex[s] < 0) mx -= f[s];
+                if (L::ey[s] > 0) my += f[s]; else if (L::ey[s] < 0) my -= f[s];
+                if (L::ez[s] > 0) mz += f[s]; else if (L::ez[s] < 0) mz -= f[s];
+            }
         }
         if constexpr (SHIFT) {
             dlt = r + (sizeof(T) == 8 ? T(a.dw_d) : T(a.dw));
@@ -512,9 +530,11 @@ __device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho,
         // set_inletW_bc_macro (D3Q19LatticeStructure.hpp:108-132)
         T s_in = T(-0.0), sz = T(-0.0);
 #pragma unroll
-        for (int i = 0; i < L::n_unk; ++i) s_in += f[L::in_list[i]];
+        for (int i = 0; i < L::n_unk; ++i)
+            s_in = SC ? fma_t(w[L::in_list[i]], f[L::in_list[i]], s_in) : s_in + f[L::in_list[i]];
 #pragma unroll
-        for (int i = 0; i < L::n_zero; ++i) sz += f[L::zero_list[i]];
+        for (int i = 0; i < L::n_zero; ++i)
+            sz = SC ? fma_t(w[L::zero_list[i]], f[L::zero_list[i]], sz) : sz + f[L::zero_list[i]];
         ux = T(0); uy = T(0);
         const T c_in0 = sizeof(T) == 8 ? T(a.c_in_d) : T(a.c_in);
         T inv_1mu = T(a.inv_1mu), c_in = c_in0;
@@ -537,9 +557,11 @@ __device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho,
         // set_outletE_bc_macro (D3Q19LatticeStructure.hpp:135-156); ux, uy kept
         T s_out = T(-0.0), sz = T(-0.0);
 #pragma unroll
-        for (int i = 0; i < L::n_unk; ++i) s_out += f[L::out_list[i]];
+        for (int i = 0; i < L::n_unk; ++i)
+            s_out = SC ? fma_t(w[L::out_list[i]], f[L::out_list[i]], s_out) : s_out + f[L::out_list[i]];
 #pragma unroll
-        for (int i = 0; i < L::n_zero; ++i) sz += f[L::zero_list[i]];
+        for (int i = 0; i < L::n_zero; ++i)
+            sz = SC ? fma_t(w[L::zero_list[i]], f[L::zero_list[i]], sz) : sz + f[L::zero_list[i]];
         rho = T(a.rho_bc);
         if constexpr (SHIFT) {
             dlt = sizeof(T) == 8 ? T(a.rho_bc - 1.0) : T(a.drho_bc);
@@ -559,7 +581,7 @@ __device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho,
             if (L::ez[s] > 0) br += (ubc - uz); else if (L::ez[s] < 0) br -= (ubc - uz);
             if (L::ex[s] > 0) br -= ux; else if (L::ex[s] < 0) br += ux;
             if (L::ey[s] > 0) br -= uy_head[s]; else if (L::ey[s] < 0) br += uy_head[s];
-            f[s] = f[s] + T(3) * rho * w[s] * br;
+            f[s] = SC ? f[s] + T(3) * rho * br : f[s] + T(3) * rho * w[s] * br;
         }
     }
     // compute_equilibrium, LatticeStructure.hpp:92-103
@@ -575,18 +597,31 @@ __device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho,
                 h = T(0.5) / rho;  // FP32X: every operation in FP64
             else
                 h = T(0.5) * __builtin_amdgcn_rcpf(rho);
-            // U = (3 + dt3) rho u: fma(3 rho, u, dt3 rho u) rounds the
-            // exact product plus the correction once, so the correction
-            // survives rounding on average; with dt3 = 0 it is 3 rho * u
-            // bit for bit (the addend is a zero of u's sign)
-            const T r3 = T(3) * rho, rd = T(a.dt3) * rho;
-            const T Ux = fma_t(r3, ux, rd * ux), Uy = fma_t(r3, uy, rd * uy), Uz = fma_t(r3, uz, rd * uz);
+            // FP16 (SC): U = (3 + dt3) rho u, as fma(3 rho, u, dt3 rho u),
+            // which rounds the exact product plus the correction once, so
+            // the correction survives rounding on average (3 + dt3 itself
+            // rounds to 3).  The other storages have dt3 = 0 and form
+            // 3 rho u as a plain product, which leaves the compiler free to
+            // contract it into the sums of t below.
+            T Ux, Uy, Uz;
+            if constexpr (SC) {
+                const T r3 = T(3) * rho, rd = T(a.dt3) * rho;
+                Ux = fma_t(r3, ux, rd * ux);
+                Uy = fma_t(r3, uy, rd * uy);
+                Uz = fma_t(r3, uz, rd * uz);
+            } else {
+                const T t3 = T(3);
+                Ux = t3 * rho * ux;
+                Uy = t3 * rho * uy;
+                Uz = t3 * rho * uz;
+            }
 #pragma unroll
             for (int s = 0; s < Q; ++s) {
                 const T t = (s == 0) ? T(0) : edot<T>(L::ex[s], L::ey[s], L::ez[s], Ux, Uy, Uz);
-                feq[s] = w[s] * (A + t * (T(1) + t * h));
+                feq[s] = SC ? A + t * (T(1) + t * h) : w[s] * (A + t * (T(1) + t * h));
             }
-            feq[0] += rho * (T(1.5) * T(a.dk1) * usq - T(a.dk0));
+            const T corr = rho * (T(1.5) * T(a.dk1) * usq - T(a.dk0));
+            feq[0] += SC ? corr * T(a.h_scale[0]) : corr;
         } else {
 #pragma unroll
             for (int s = 0; s < Q; ++s) {
@@ -608,13 +643,15 @@ __device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho,
 }
 
 // Non-equilibrium stress Pi = sum_s e e (f - feq) (compute_strain_tensor,
-// LatticeStructure.hpp:105-122): xx, yy, zz, xy, xz, yz.
-template <class L, class T>
-__device__ inline void node_stress(const T (&f)[L::Q], const T (&feq)[L::Q], T (&pi)[6]) {
+// LatticeStructure.hpp:105-122): xx, yy, zz, xy, xz, yz.  SC: f and feq
+// hold h = d / w, and each difference is weighed by its w.
+template <class L, class T, bool SC = false>
+__device__ inline void node_stress(const T (&f)[L::Q], const T (&feq)[L::Q], T (&pi)[6],
+                                   const T (&w)[L::Q]) {
     T sxx = T(-0.0), syy = T(-0.0), szz = T(-0.0), sxy = T(-0.0), sxz = T(-0.0), syz = T(-0.0);
 #pragma unroll
     for (int s = 0; s < L::Q; ++s) {
-        const T fn = f[s] - feq[s];
+        const T fn = SC ? (f[s] - feq[s]) * w[s] : f[s] - feq[s];
         if (L::ex[s] != 0) sxx += fn;
         if (L::ey[s] != 0) syy += fn;
         if (L::ez[s] != 0) szz += fn;
@@ -695,18 +732,20 @@ template <int ST, int Q>
 constexpr bool kConserve = ST == ST_F32;
 
 // The whole reference update of one node (TLBM_Partition.cpp:742-807).
-// CONSERVE: fluid nodes relax with node_relax_conserving.
-template <class L, class T, bool SHIFT, bool COLLIDE, bool CONSERVE = false>
+// CONSERVE: fluid nodes relax with node_relax_conserving.  SC: the scaled
+// form of node_prepare (the relaxation is the same in either scale).
+template <class L, class T, bool SHIFT, bool COLLIDE, bool CONSERVE = false, bool SC = false>
 __device__ inline void node_update(T (&f)[L::Q], int t, T &rho, T &ux, T &uy, T &uz,
                                    const StepArgs &a, const T *__restrict__ uy_head,
                                    const T (&w)[L::Q], uint32_t g) {
+    static_assert(!(SC && CONSERVE), "the conserving relax works on d");
     T feq[L::Q];
-    if (!node_prepare<L, T, SHIFT, COLLIDE>(f, feq, t, rho, ux, uy, uz, a, uy_head, w, g)) return;
+    if (!node_prepare<L, T, SHIFT, COLLIDE, SC>(f, feq, t, rho, ux, uy, uz, a, uy_head, w, g)) return;
     if (!COLLIDE) return;
     T om = T(a.omega);
     if (a.smag) {
         T pi[6];
-        node_stress<L, T>(f, feq, pi);
+        node_stress<L, T, SC>(f, feq, pi, w);
         om = smagorinsky_omega<T>(pi, a);
     }
     if (a.dynamics == 1) {
@@ -787,7 +826,8 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
     }
     T rho, ux, uy, uz;
     if (MODE == MODE_STEP) {
-        node_update<L, T, S::kShift, true, kConserve<ST, Q>>(f, t, rho, ux, uy, uz, a, uy_head, w, g);
+        node_update<L, T, S::kShift, true, kConserve<ST, Q>, S::kScaled>(f, t, rho, ux, uy, uz, a,
+                                                                         uy_head, w, g);
         const PushOut<L, ST> po(fout, (blk * BS) >> StorageT<ST>::kCB, g);
         const uint32_t rs = ST == ST_F16 ? round_stream(g, a.step) : 0u;
 #pragma unroll
@@ -796,9 +836,10 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
     } else if (MODE == MODE_STRESS) {
         T feq[Q], pi[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
         T c = T(0);
-        if (node_prepare<L, T, S::kShift, true>(f, feq, t, rho, ux, uy, uz, a, uy_head, w, g) &&
+        if (node_prepare<L, T, S::kShift, true, S::kScaled>(f, feq, t, rho, ux, uy, uz, a, uy_head,
+                                                            w, g) &&
             a.dynamics == 1) {
-            node_stress<L, T>(f, feq, pi);
+            node_stress<L, T, S::kScaled>(f, feq, pi, w);
             const T om = a.smag ? smagorinsky_omega<T>(pi, a) : T(a.omega);
             c = T(0.5) * om - T(1);
         }
@@ -810,7 +851,7 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
 #pragma unroll
         for (int k = 0; k < 6; ++k) macro[size_t(4 + k) * a.N + g] = c * pi[k];
     } else {
-        node_update<L, T, S::kShift, false>(f, t, rho, ux, uy, uz, a, uy_head, w, g);
+        node_update<L, T, S::kShift, false, false, S::kScaled>(f, t, rho, ux, uy, uz, a, uy_head, w, g);
         if (MODE == MODE_MACROS) {
             T *macro = static_cast<T *>(out);
             macro[g] = ux;
@@ -887,9 +928,10 @@ k_lbm_closure(const typename StorageT<ST>::st *__restrict__ fin, typename Storag
         t = nd_type[g];
     }
     T feq[Q], rho, ux, uy, uz;
-    const bool relaxes = node_prepare<L, T, S::kShift, true>(f, feq, t, rho, ux, uy, uz, a, uy_head, w, g);
+    const bool relaxes = node_prepare<L, T, S::kShift, true, S::kScaled>(f, feq, t, rho, ux, uy, uz,
+                                                                         a, uy_head, w, g);
     T pi[6];
-    node_stress<L, T>(f, feq, pi);
+    node_stress<L, T, S::kScaled>(f, feq, pi, w);
     float feat[kMlpIn];
 #pragma unroll
     for (int k = 0; k < 6; ++k) feat[k] = float(pi[k]) * a.cl_in;

@@ -255,6 +255,10 @@ int tlbm_gpu_create(const tlbm_params *p, const uint8_t *nd_type, tlbm_gpu **out
         // of lattices too large for 32-bit relative offsets (for tests)
         const char *gen = std::getenv("TLBM_GENERIC_ADDRESSING");
         if (gen && gen[0] == '1') a.rel = 0;
+        // rows never straddle a chunk: the source row's chunk is the source's
+        // (TLBM_ROW_ADDRESSING=0: the per-population relative gather, for tests)
+        const char *rowa = std::getenv("TLBM_ROW_ADDRESSING");
+        a.row_chunks = a.rel && (CH % a.nx == 0) && !(rowa && rowa[0] == '0');
     }
     switch (Q) {
     case 15: fill_weights<tlbm::D3Q15>(a, p->exact_weights != 0, p->storage); break;
