@@ -119,6 +119,22 @@ int tlbm_gpu_init_from_macros(tlbm_gpu *h, const double *ux, const double *uy, c
  * or divergence checks (its loop, src/tLBM.cpp:45-62, steps on). */
 int tlbm_gpu_reduce(tlbm_gpu *h, double out[5]);
 
+/* The sums behind tlbm_gpu_reduce, written to device memory and queued on
+ * the handle's stream: out_dev (device pointer, TLBM_REDUCE_SUMS doubles)
+ * gets sum f, sum f*e (x, y, z), sum (B_n - B_{n-1})^2 (0 before the first
+ * step), sum B_n^2 and the number of NaN or infinite populations.  It
+ * allocates nothing and does not synchronise (read out_dev after
+ * tlbm_gpu_sync): its partial sums live in a fixed device array of
+ * 1024 x 7 doubles (56 KiB per GPU, held until the process ends), and a
+ * lattice whose reduction needs more partials is rejected (TLBM_EINVAL;
+ * none does today: the partial count is capped at 1024).  Every handle on
+ * a GPU shares that GPU's array, so calls whose reductions may run at the
+ * same time on one GPU (other threads, other handles' streams) race.
+ * No counterpart in the reference: for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops). */
+#define TLBM_REDUCE_SUMS 7
+int tlbm_gpu_reduce_device(tlbm_gpu *h, double *out_dev);
+
 /* enable = 1: zero the time-average sums and from then on add ux, uy, uz
  * and rho of every step to them on the device; enable = 0: stop adding.
  * TLBM_Partition::update_time_avg (src/TLBM_Partition.cpp:815-824) driven

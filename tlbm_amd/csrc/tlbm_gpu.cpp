@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -14,6 +15,18 @@
 
 namespace {
 thread_local std::string g_err;
+
+// Partial sums of tlbm_gpu_reduce_device: a fixed device array, so that
+// entry point allocates nothing.  Each GPU holds its own copy until the
+// process ends; every handle on that GPU shares it.  In host code the
+// array's name is not a device address: its address on each device comes
+// from hipGetSymbolAddress (g_scratch_ptr, resolved once per device).
+constexpr int kScratchParts = 1024;
+static_assert(TLBM_REDUCE_SUMS == tlbm::kRed, "sums of the FP64 reduction");
+__device__ __align__(256) double scratch[kScratchParts * tlbm::kRed];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
 }  // namespace
 
 namespace tlbm {
@@ -136,14 +149,15 @@ struct tlbm_gpu {
     void *macro = nullptr;    // 4*N compute-type values (dumps)
     void *filt_in = nullptr;  // 10*N compute-type values (box filter input)
     void *uy_head = nullptr;  // Q compute-type values (type-5 rule)
-    double *scratch = nullptr;  // staging for forces / populations / macros (lazy)
-    size_t scratch_bytes = 0;
+    double *staging = nullptr;  // staging for forces / populations / macros (lazy)
+    size_t staging_bytes = 0;
     double *partial = nullptr;  // reduction partials
     int nparts = 0;
     bool has_lid = false;
     long long steps_since_set = 0;
     uint64_t step_count = 0;  // steps since creation (FP16 rounding stream)
     hipStream_t stream = nullptr;
+    double *fixed_partial = nullptr;  // this device's copy of `scratch`
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     uint64_t bytes = 0;
     std::vector<uint8_t> chunk_types;  // host copy, kept until the upload completes
@@ -156,22 +170,39 @@ struct tlbm_gpu {
 
 namespace {
 
-int ensure_scratch(tlbm_gpu *h, size_t bytes) {
-    if (h->scratch_bytes >= bytes) return TLBM_OK;
-    if (h->scratch) {
-        TLBM_HIP(hipFree(h->scratch));
-        h->bytes -= h->scratch_bytes;
-        h->scratch = nullptr;
-        h->scratch_bytes = 0;
+int ensure_staging(tlbm_gpu *h, size_t bytes) {
+    if (h->staging_bytes >= bytes) return TLBM_OK;
+    if (h->staging) {
+        TLBM_HIP(hipFree(h->staging));
+        h->bytes -= h->staging_bytes;
+        h->staging = nullptr;
+        h->staging_bytes = 0;
     }
-    TLBM_HIP(hipMalloc(&h->scratch, bytes));
-    h->scratch_bytes = bytes;
+    TLBM_HIP(hipMalloc(&h->staging, bytes));
+    h->staging_bytes = bytes;
     h->bytes += bytes;
     return TLBM_OK;
 }
 
 int check(const tlbm_gpu *h) {
     if (!h) return fail(TLBM_EINVAL, "null handle");
+    return TLBM_OK;
+}
+
+// The current device's address of `scratch`, resolved with
+// hipGetSymbolAddress on the first handle created on that device.
+int fixed_partials(double **out) {
+    int dev = 0;
+    TLBM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(TLBM_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        TLBM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(TLBM_EHIP, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = static_cast<double *>(g_scratch_ptr[dev]);
     return TLBM_OK;
 }
 
@@ -269,6 +300,7 @@ int tlbm_gpu_create(const tlbm_params *p, const uint8_t *nd_type, tlbm_gpu **out
         tlbm_gpu_destroy(h);
         return code;
     };
+    if (int rc = fixed_partials(&h->fixed_partial)) return cleanup_fail(rc);
     hipError_t e;
     if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
         return cleanup_fail(fail(TLBM_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
@@ -335,7 +367,7 @@ int tlbm_gpu_destroy(tlbm_gpu *h) {
     if (h->macro) (void)hipFree(h->macro);
     if (h->filt_in) (void)hipFree(h->filt_in);
     if (h->uy_head) (void)hipFree(h->uy_head);
-    if (h->scratch) (void)hipFree(h->scratch);
+    if (h->staging) (void)hipFree(h->staging);
     if (h->partial) (void)hipFree(h->partial);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
@@ -434,12 +466,12 @@ int tlbm_gpu_get_macros(tlbm_gpu *h, double *ux, double *uy, double *uz, double 
 int tlbm_gpu_get_forces(tlbm_gpu *h, double *fx, double *fy, double *fz) {
     if (int rc = check(h)) return rc;
     if (!fx || !fy || !fz) return fail(TLBM_EINVAL, "null output");
-    if (int rc = ensure_scratch(h, size_t(3) * h->N * sizeof(double))) return rc;
-    hipError_t e = h->ops->forces(h->Q, h->f[h->cur], h->nd, h->scratch, h->args, h->stream);
+    if (int rc = ensure_staging(h, size_t(3) * h->N * sizeof(double))) return rc;
+    hipError_t e = h->ops->forces(h->Q, h->f[h->cur], h->nd, h->staging, h->args, h->stream);
     if (e != hipSuccess) return fail(TLBM_EHIP, std::string("forces: ") + hipGetErrorString(e));
     double *dst[3] = {fx, fy, fz};
     for (int k = 0; k < 3; ++k)
-        TLBM_HIP(hipMemcpyAsync(dst[k], h->scratch + size_t(k) * h->N, size_t(h->N) * 8,
+        TLBM_HIP(hipMemcpyAsync(dst[k], h->staging + size_t(k) * h->N, size_t(h->N) * 8,
                                 hipMemcpyDeviceToHost, h->stream));
     TLBM_HIP(hipStreamSynchronize(h->stream));
     return TLBM_OK;
@@ -449,10 +481,10 @@ int tlbm_gpu_get_populations(tlbm_gpu *h, double *f) {
     if (int rc = check(h)) return rc;
     if (!f) return fail(TLBM_EINVAL, "null output");
     const size_t b = size_t(h->N) * h->Q * sizeof(double);
-    if (int rc = ensure_scratch(h, b)) return rc;
-    hipError_t e = h->ops->get_populations(h->Q, h->f[h->cur], h->scratch, h->args, h->stream);
+    if (int rc = ensure_staging(h, b)) return rc;
+    hipError_t e = h->ops->get_populations(h->Q, h->f[h->cur], h->staging, h->args, h->stream);
     if (e != hipSuccess) return fail(TLBM_EHIP, std::string("get populations: ") + hipGetErrorString(e));
-    TLBM_HIP(hipMemcpyAsync(f, h->scratch, b, hipMemcpyDeviceToHost, h->stream));
+    TLBM_HIP(hipMemcpyAsync(f, h->staging, b, hipMemcpyDeviceToHost, h->stream));
     TLBM_HIP(hipStreamSynchronize(h->stream));
     return TLBM_OK;
 }
@@ -461,10 +493,10 @@ int tlbm_gpu_set_populations(tlbm_gpu *h, const double *f) {
     if (int rc = check(h)) return rc;
     if (!f) return fail(TLBM_EINVAL, "null input");
     const size_t b = size_t(h->N) * h->Q * sizeof(double);
-    if (int rc = ensure_scratch(h, b)) return rc;
-    TLBM_HIP(hipMemcpyAsync(h->scratch, f, b, hipMemcpyHostToDevice, h->stream));
+    if (int rc = ensure_staging(h, b)) return rc;
+    TLBM_HIP(hipMemcpyAsync(h->staging, f, b, hipMemcpyHostToDevice, h->stream));
     for (int i = 0; i < 2; ++i) {
-        hipError_t e = h->ops->set_populations(h->Q, h->f[i], h->scratch, h->args, h->stream);
+        hipError_t e = h->ops->set_populations(h->Q, h->f[i], h->staging, h->args, h->stream);
         if (e != hipSuccess) return fail(TLBM_EHIP, std::string("set populations: ") + hipGetErrorString(e));
     }
     TLBM_HIP(hipStreamSynchronize(h->stream));
@@ -477,13 +509,13 @@ int tlbm_gpu_init_from_macros(tlbm_gpu *h, const double *ux, const double *uy, c
     if (int rc = check(h)) return rc;
     if (!ux || !uy || !uz || !rho) return fail(TLBM_EINVAL, "null input");
     const size_t plane = size_t(h->N) * sizeof(double);
-    if (int rc = ensure_scratch(h, 4 * plane)) return rc;
+    if (int rc = ensure_staging(h, 4 * plane)) return rc;
     const double *src[4] = {ux, uy, uz, rho};
     for (int k = 0; k < 4; ++k)
-        TLBM_HIP(hipMemcpyAsync(reinterpret_cast<char *>(h->scratch) + k * plane, src[k], plane,
+        TLBM_HIP(hipMemcpyAsync(reinterpret_cast<char *>(h->staging) + k * plane, src[k], plane,
                                 hipMemcpyHostToDevice, h->stream));
     for (int i = 0; i < 2; ++i) {
-        hipError_t e = h->ops->init_macros(h->Q, h->f[i], h->scratch, h->args, h->stream);
+        hipError_t e = h->ops->init_macros(h->Q, h->f[i], h->staging, h->args, h->stream);
         if (e != hipSuccess) return fail(TLBM_EHIP, std::string("init from macros: ") + hipGetErrorString(e));
     }
     TLBM_HIP(hipStreamSynchronize(h->stream));
@@ -514,6 +546,22 @@ int tlbm_gpu_reduce(tlbm_gpu *h, double out[5]) {
                       r[6], (long long)h->step_count);
         return fail(TLBM_ENONFINITE, buf);
     }
+    return TLBM_OK;
+}
+
+int tlbm_gpu_reduce_device(tlbm_gpu *h, double *out_dev) {
+    if (int rc = check(h)) return rc;
+    if (!out_dev) return fail(TLBM_EINVAL, "null output");
+    if (h->nparts > kScratchParts)
+        return fail(TLBM_EINVAL, "the reduction needs " + std::to_string(h->nparts) +
+                                     " partial sums; the fixed array holds " +
+                                     std::to_string(kScratchParts));
+    // the partial kernel writes every one of the nparts partials before the
+    // final kernel reads them, so nothing needs clearing between calls
+    const void *prev = h->steps_since_set >= 1 ? h->f[1 - h->cur] : nullptr;
+    hipError_t e = h->ops->reduce(h->Q, h->f[h->cur], prev, h->fixed_partial, h->nparts, out_dev,
+                                  h->args, h->stream);
+    if (e != hipSuccess) return fail(TLBM_EHIP, std::string("reduce: ") + hipGetErrorString(e));
     return TLBM_OK;
 }
 
@@ -619,10 +667,10 @@ int tlbm_gpu_box_filter(tlbm_gpu *h, double *out) {
     const size_t nc = size_t(h->args.nx / 2) * (h->args.ny / 2) * (h->args.nz / 2);
     static_assert(tlbm::kFilterPlanes == TLBM_FILTER_PLANES, "box filter planes");
     const size_t ob = size_t(tlbm::kFilterPlanes) * nc * sizeof(double);
-    if (int rc = ensure_scratch(h, ob)) return rc;
-    e = h->ops->box_filter(h->filt_in, h->scratch, h->args, h->stream);
+    if (int rc = ensure_staging(h, ob)) return rc;
+    e = h->ops->box_filter(h->filt_in, h->staging, h->args, h->stream);
     if (e != hipSuccess) return fail(TLBM_EHIP, std::string("box filter: ") + hipGetErrorString(e));
-    TLBM_HIP(hipMemcpyAsync(out, h->scratch, ob, hipMemcpyDeviceToHost, h->stream));
+    TLBM_HIP(hipMemcpyAsync(out, h->staging, ob, hipMemcpyDeviceToHost, h->stream));
     TLBM_HIP(hipStreamSynchronize(h->stream));
     return TLBM_OK;
 }

@@ -60,6 +60,7 @@ def load_library(path: str = LIB_PATH):
     L.tlbm_gpu_set_populations.argtypes = [_P, _P]
     L.tlbm_gpu_init_from_macros.argtypes = [_P, _P, _P, _P, _P]
     L.tlbm_gpu_reduce.argtypes = [_P, _P]
+    L.tlbm_gpu_reduce_device.argtypes = [_P, _P]
     L.tlbm_gpu_memory_bytes.argtypes = [_P, ctypes.POINTER(ctypes.c_uint64)]
     L.tlbm_gpu_time_average.argtypes = [_P, ctypes.c_int]
     L.tlbm_gpu_get_time_average.argtypes = [_P, _P, _P, _P, _P, ctypes.POINTER(ctypes.c_int64)]
@@ -82,7 +83,7 @@ def load_library(path: str = LIB_PATH):
     L.tlbm_gpu_last_error.restype = ctypes.c_char_p
     L.tlbm_gpu_last_error.argtypes = []
     for name in ("create", "destroy", "step", "step_timed", "sync", "get_macros", "get_forces",
-                 "get_populations", "set_populations", "init_from_macros", "reduce",
+                 "get_populations", "set_populations", "init_from_macros", "reduce", "reduce_device",
                  "memory_bytes", "time_average", "get_time_average"):
         getattr(L, f"tlbm_gpu_{name}").restype = ctypes.c_int
     _lib = L
