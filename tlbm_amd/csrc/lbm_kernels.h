@@ -51,6 +51,14 @@ constexpr int kClosureBlock = 512;  // workgroup of the closure collide
 // non-temporal population loads in the closure collide: FP16 only (FP32:
 // 221.8 against 217.6 us per step with them)
 template <int ST> constexpr bool kClosureNTLoad = (ST == ST_F16);
+// waves per SIMD the closure collide is register-allocated for.  The
+// compiler alone stops at 4 (3 for D3Q27); more waves hide the latency of
+// the network's MFMA chain.  Measured on 128x128x512 (us per step, 4 waves ->
+// this): FP16 D3Q19 156.5 -> 149.1 (6), D3Q15 141.9 -> 130.8 (6), D3Q27
+// 316.8 -> 197.6 (5); FP32 D3Q19 216.5 -> 214.5 (5), D3Q15 181.9 -> 167.8
+// (6).  FP32 D3Q27 stays at 4: at 5 it spills (325 against 299 us).
+template <int ST, int Q>
+constexpr int kClosureWaves = Q == 27 ? (ST == ST_F16 ? 5 : 4) : (ST == ST_F32 && Q == 19 ? 5 : 6);
 
 // Launch arguments, passed by value (kernarg segment, scalar registers).
 struct StepArgs {
@@ -644,14 +652,19 @@ __device__ inline bool node_prepare(T (&f)[L::Q], T (&feq)[L::Q], int t, T &rho,
 
 // Non-equilibrium stress Pi = sum_s e e (f - feq) (compute_strain_tensor,
 // LatticeStructure.hpp:105-122): xx, yy, zz, xy, xz, yz.  SC: f and feq
-// hold h = d / w, and each difference is weighed by its w.
+// hold h = d / w, and each difference is weighed by its w; a direction and
+// its partner (bounce-back table) have the same e e and the same w, so
+// their differences are added first and each pair enters a sum once (about
+// a third fewer operations than one term per direction).
 template <class L, class T, bool SC = false>
 __device__ inline void node_stress(const T (&f)[L::Q], const T (&feq)[L::Q], T (&pi)[6],
                                    const T (&w)[L::Q]) {
     T sxx = T(-0.0), syy = T(-0.0), szz = T(-0.0), sxy = T(-0.0), sxz = T(-0.0), syz = T(-0.0);
 #pragma unroll
     for (int s = 0; s < L::Q; ++s) {
-        const T fn = SC ? (f[s] - feq[s]) * w[s] : f[s] - feq[s];
+        // the rest direction adds to no sum; the others come with their partner
+        if (SC && (s == 0 || L::bb[s] < s)) continue;
+        const T fn = SC ? ((f[s] - feq[s]) + (f[L::bb[s]] - feq[L::bb[s]])) * w[s] : f[s] - feq[s];
         if (L::ex[s] != 0) sxx += fn;
         if (L::ey[s] != 0) syy += fn;
         if (L::ez[s] != 0) szz += fn;
@@ -875,7 +888,7 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
 // together (lanes past the last node work on the last node and store
 // nothing).  A solid node's stress is zero and its result unused.
 template <class L, int ST, int BS, bool NTLOAD>
-__global__ void __launch_bounds__(BS)
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(kClosureWaves<ST, L::Q>)))
 k_lbm_closure(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::st *__restrict__ fout,
               const uint8_t *__restrict__ nd_type, const uint8_t *__restrict__ chunk_type,
               const typename StorageT<ST>::T *__restrict__ uy_head, StepArgs a) {
@@ -943,11 +956,13 @@ k_lbm_closure(const typename StorageT<ST>::st *__restrict__ fin, typename Storag
     __syncthreads();  // smlp complete (every lane of the workgroup gets here)
     const float zz = mlp_wave(feat, &smlp);
     const float om = __builtin_amdgcn_rcpf(a.tau0 + a.cl_out * softplus_f(zz));  // 1 ulp
-    if (relaxes && a.dynamics == 1) {
+    if (a.dynamics == 1) {
         if (kConserve<ST, Q> && t == 0)
             node_relax_conserving<L>(f, feq, w, om, rho, ux, uy, uz, a);
         else
-            node_relax<L, T>(f, feq, T(om));
+            // the one node type that does not relax, the solid node, has
+            // feq = f, so the rate 0 leaves it exactly as it is; no branch
+            node_relax<L, T>(f, feq, relaxes ? T(om) : T(0));
     }
     if (!live) return;
     const PushOut<L, ST> po(fout, (blockIdx.x * BS) >> StorageT<ST>::kCB, g);

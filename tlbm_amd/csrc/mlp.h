@@ -26,8 +26,9 @@
 // layer 1's registers 8kb .. 8kb+7 (units 16kb + 4g + r and
 // 16kb + 8 + 4g + r, r = 0..3), and W2 is packed in that order.  So
 // layer 1's output feeds layer 2 in registers.  The only cross-lane moves
-// are the 7 features (lane l <-> l^32) and the final sum of the two lane
-// halves.  Weights, features and h1 are rounded to bf16 (round to nearest
+// are four v_permlane32_swap of the features (they build both blocks' B
+// operands at once) and one for the final sum of the two lane halves; no
+// LDS round trip.  Weights, features and h1 are rounded to bf16 (round to nearest
 // even); biases, h2 and the last layer are FP32.  The whole wavefront must
 // execute mlp_wave together.
 #pragma once
@@ -45,7 +46,7 @@ constexpr int kMlpParams = kMlpH * kMlpIn + kMlpH + kMlpH * kMlpH + kMlpH + kMlp
 // values a lane reads from LDS sit at small constant offsets from one base
 // per lane half (hb[h]) or per lane (a1[l], a2[kb][l]), so the reads take
 // their offsets as immediates instead of an address computation each.
-struct MlpHalf {
+struct alignas(16) MlpHalf {
     float b1[16];  // b1 of unit 8q+4h+r at [4q+r]
     float b2[16];
     float w3[16];  // w3 rounded to bf16
@@ -157,6 +158,28 @@ __device__ inline bf16x8 load_quad(const uint32_t (&w)[4]) {
     return __builtin_bit_cast(bf16x8, v);
 }
 
+// 0, as a value the compiler cannot see through (an empty asm statement
+// owns it), so that loads through it are not merged with the same loads
+// elsewhere.
+__device__ inline uint32_t opaque_zero() {
+    uint32_t z = 0;
+    __asm__ volatile("" : "+s"(z));
+    return z;
+}
+
+// v_permlane32_swap_b32: lanes 32-63 of a trade places with lanes 0-31 of b.
+// Afterwards lane l < 32 of b holds a's lane l + 32 and lane l >= 32 of a
+// holds b's lane l - 32; the other halves stay.
+__device__ inline void swap_halves(float &a, float &b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__builtin_bit_cast(uint32_t, a),
+                                                    __builtin_bit_cast(uint32_t, b), false, false);
+    // the elements go through named scalars: this clang reads
+    // __builtin_bit_cast(float, r[1]) from r's first element
+    const uint32_t r0 = r[0], r1 = r[1];
+    a = __builtin_bit_cast(float, r0);
+    b = __builtin_bit_cast(float, r1);
+}
+
 __device__ inline f32x16 load16(const float (&v)[16]) {
     f32x16 r;
 #pragma unroll
@@ -174,18 +197,22 @@ __device__ inline f32x16 load16(const float (&v)[16]) {
 __device__ inline float mlp_wave(const float (&x)[kMlpIn], const MlpPacked *__restrict__ m) {
     const int l = int(threadIdx.x & 63u);
     const int h = l >> 5;
-    float xo[kMlpIn];
-#pragma unroll
-    for (int k = 0; k < kMlpIn; ++k) xo[k] = __shfl_xor(x[k], 32);
     // layer-1 B operand: block 0 = nodes of lanes 0-31, block 1 = lanes 32-63;
-    // lane l supplies inputs 4h..4h+3 of node l%32 of each block
-    const bf16x4 xb[2] = {h ? pack4(xo[4], xo[5], xo[6], 0.f) : pack4(x[0], x[1], x[2], x[3]),
-                          h ? pack4(x[4], x[5], x[6], 0.f) : pack4(xo[0], xo[1], xo[2], xo[3])};
+    // lane l supplies inputs 4h..4h+3 of node l%32 of each block.  Swapping
+    // the upper half of (x0..x3) with the lower half of (x4..x6, 0) leaves
+    // block 0's operand in the first and block 1's in the second.
+    float lo[4] = {x[0], x[1], x[2], x[3]}, hi[4] = {x[4], x[5], x[6], 0.f};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) swap_halves(lo[k], hi[k]);
+    const bf16x4 xb[2] = {pack4(lo[0], lo[1], lo[2], lo[3]), pack4(hi[0], hi[1], hi[2], hi[3])};
     const bf16x4 a1 = load_pair(m->a1[l]);
     float part[2];
 #pragma unroll
     for (int blk = 0; blk < 2; ++blk) {
-        const MlpHalf &H = m->hb[h];
+        // each block reads its biases afresh (opaque_zero): they go straight
+        // into the accumulators that the MFMAs then overwrite, instead of
+        // staying in registers for the other block
+        const MlpHalf &H = m->hb[h + int(opaque_zero())];
         const f32x16 c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a1, xb[blk], load16(H.b1), 0, 0, 0);
         f32x16 d = load16(H.b2);
 #pragma unroll
@@ -200,9 +227,9 @@ __device__ inline float mlp_wave(const float (&x)[kMlpIn], const MlpPacked *__re
         for (int r = 0; r < 16; ++r) p += H.w3[r] * relu(d[r]);
         part[blk] = p;
     }
-    const float pa = part[0] + __shfl_xor(part[0], 32);
-    const float pb = part[1] + __shfl_xor(part[1], 32);
-    return (h ? pb : pa) + m->b3;
+    // block 0's halves meet in the lower lanes, block 1's in the upper ones
+    swap_halves(part[0], part[1]);
+    return (part[0] + part[1]) + m->b3;
 }
 
 // softplus with the hardware exp2/log2 (v_exp_f32, v_log_f32, 1 ulp each):
