@@ -16,11 +16,14 @@ Smagorinsky's constant cs in {0.025, 0.05, 0.1} and the same base offsets
     closer to the filtered fine run with the network after 50, 200 and 400
     coarse steps;
   - the mean velocity over the held-out window of a long coarse run from
-    uniform flow is as close with either closure, within 5 %.  The network
-    does not win here: the difference between the two (about 2 %) is of the
-    size by which rounding-level changes move this statistic (its sign
-    flipped with a change in the FP32 kernels' contraction), and a 1e-7
-    perturbation of the inlet moves it too (measured beside it).
+    uniform flow is as close with either closure, within 5 %.  This does
+    not claim a win: the network's lead (0.0506 against 0.0519) is smaller
+    than the 4 % by which a rounding-level change of the FP32 kernels (the
+    contraction of 3 rho u into the equilibrium's sums), carried through the
+    whole pipeline (fine run, training, calibration), once moved the
+    network's figure (to 0.0529).  A 1e-7 perturbation of the coarse
+    inlet, run beside it, moves Smagorinsky's figure by only 2e-8: the
+    coarse long run itself is not sensitive, the trained network is.
 
 The run is deterministic (seeded inlet, seeded sampling, deterministic
 kernels), so the margins are fixed numbers, not draws.
