@@ -134,7 +134,7 @@ struct Launch {
     template <class L>
     static hipError_t reduce_l(const void *cur, const void *prev, double *partial, int nparts,
                                double *out, const StepArgs &a, hipStream_t s) {
-        hipLaunchKernelGGL((k_reduce_partial<L, ST>), dim3(nparts), dim3(kBlock), 0, s,
+        hipLaunchKernelGGL((k_reduce_partial<L, ST>), dim3(nparts), dim3(kRedBlock), 0, s,
                            static_cast<const st *>(cur), static_cast<const st *>(prev), partial, a);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -207,11 +207,12 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-// The kRed sums of a kBlock-thread workgroup: wavefront sums, then the
-// kBlock/64 wave totals folded in wave order by threads 0..kRed-1, which
+// The kRed sums of a BS-thread workgroup: wavefront sums, then the
+// BS/64 wave totals folded in wave order by threads 0..kRed-1, which
 // return them (other threads return zeros).  Deterministic.
+template <int BS>
 __device__ inline void block_sums(double (&acc)[kRed]) {
-    constexpr int W = kBlock / 64;
+    constexpr int W = BS / 64;
     __shared__ double sh[kRed][W];
     const int lane = int(threadIdx.x & 63u), wave = int(threadIdx.x >> 6);
 #pragma unroll
@@ -231,15 +232,21 @@ __device__ inline void block_sums(double (&acc)[kRed]) {
 
 // Per-workgroup FP64 partial sums over the state: sum f, sum f e_x,
 // sum f e_y, sum f e_z, sum (B_n - B_{n-1})^2, sum B_n^2 and the number of
-// populations that are NaN or infinite (kRed values).
+// populations that are NaN or infinite (kRed values).  Workgroups of
+// kRedBlock threads: each thread streams 2 Q values per node through a few
+// FP64 chains, and the loads of one node only overlap with other waves',
+// so the partials run as many waves as their registers allow (D3Q19: 8 per
+// SIMD; 1024x512x512 FP16: 18.5 ms with 256-thread workgroups, 10.4 ms
+// with 1024).
+constexpr int kRedBlock = 1024;
 template <class L, int ST>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kRedBlock)
 k_reduce_partial(const typename StorageT<ST>::st *__restrict__ cur,
                  const typename StorageT<ST>::st *__restrict__ prev, double *__restrict__ partial,
                  StepArgs a) {
     using S = StorageT<ST>;
     double acc[kRed] = {0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t g = blockIdx.x * kBlock + threadIdx.x; g < a.N; g += gridDim.x * kBlock) {
+    for (uint32_t g = blockIdx.x * kRedBlock + threadIdx.x; g < a.N; g += gridDim.x * kRedBlock) {
 #pragma unroll
         for (int s = 0; s < L::Q; ++s) {
             const double v = S::absolute(S::load(cur + fidx<L::Q>(s, g, a), s, a), s, a);
@@ -255,7 +262,7 @@ k_reduce_partial(const typename StorageT<ST>::st *__restrict__ cur,
             }
         }
     }
-    block_sums(acc);
+    block_sums<kRedBlock>(acc);
     if (threadIdx.x < kRed) partial[size_t(blockIdx.x) * kRed + threadIdx.x] = acc[0];
 }
 
@@ -269,7 +276,7 @@ k_reduce_final(const double *__restrict__ partial, int nparts, double *__restric
 #pragma unroll
         for (int k = 0; k < kRed; ++k) acc[k] += partial[size_t(i) * kRed + k];
     }
-    block_sums(acc);
+    block_sums<kBlock>(acc);
     if (threadIdx.x < kRed) out[threadIdx.x] = acc[0];
 }
 
