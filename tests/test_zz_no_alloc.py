@@ -72,7 +72,7 @@ def _agree(out, rc, dev, first_step):
     assert (dev[6] == 0) == (rc == 0)
 
 
-@pytest.mark.parametrize("storage", ["f64", "f32", "f16"])
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16", "f32x"])
 def test_reduce_device_matches_reduce_two_calls(gpu_lib, hip, storage):
     buf = DeviceBuffer(hip)
     try:
@@ -96,6 +96,15 @@ def test_reduce_device_matches_reduce_two_calls(gpu_lib, hip, storage):
             _agree(out, rc, dev, False)
             assert rc == 0
             # the mass is the small lattice's own, not a leftover of the first
+            assert abs(dev[0] / (nx * ny * nz) - 1.0) < 0.05
+        # and a D3Q27 lattice after them (its own partial kernel)
+        nx, ny, nz = 24, 20, 32
+        nd = geometry.channel(nx, ny, nz, obstacles=[geometry.sphere(12, 10, 12, 4)])
+        with Lattice(27, nx, ny, nz, nd, storage=storage, omega=1.5, u_lbm=0.03) as lat:
+            lat.step(5)
+            out, rc, dev = _both(lat, buf)
+            _agree(out, rc, dev, False)
+            assert rc == 0
             assert abs(dev[0] / (nx * ny * nz) - 1.0) < 0.05
     finally:
         buf.free()
