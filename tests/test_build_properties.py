@@ -98,3 +98,14 @@ def test_closure_collide_holds_its_waves(kernels, lattice, storage, waves):
     assert len(name) == 1, name
     meta = kernels[name[0]]
     assert _waves(meta["vgpr_count"], meta.get("agpr_count", 0)) >= waves, meta
+
+
+@pytest.mark.parametrize("storage,waves", [(0, 4), (1, 5), (2, 8), (3, 4)])
+def test_d3q19_step_holds_its_waves(kernels, storage, waves):
+    """The D3Q19 step kernels run the occupancy they were measured at
+    (FP64 4 waves per SIMD, one 1024-thread workgroup per CU; FP32 5; FP32X,
+    whose arithmetic is FP64, 4; FP16 8)."""
+    name = [k for k in kernels if re.match(rf"_ZN4tlbm5k_lbmINS_5D3Q19ELi{storage}ELi0ELb0E", k)]
+    assert len(name) == 1, name
+    meta = kernels[name[0]]
+    assert _waves(meta["vgpr_count"], meta.get("agpr_count", 0)) >= waves, meta
