@@ -683,15 +683,15 @@ def evaluate(case: ClosureCase, nd_fine, data, closures: dict, *, starts=4,
     return out
 
 
-def step_cost(shape, closure, storage="f32", steps=200, warmup=20, repeats=3):
+def step_cost(shape, closure, storage="f32", steps=200, warmup=20, repeats=3, q=19):
     """Milliseconds per step of the Smagorinsky and the network collide on a
-    lattice of `shape` (sphere-free channel, periodic x/y)."""
+    D3Q`q` lattice of `shape` (sphere-free channel, periodic x/y)."""
     nx, ny, nz = shape
     nd = geometry.channel(nx, ny, nz, walls=())
     res = {}
     for name in ("smagorinsky", "network"):
         cs = 0.1 if name == "smagorinsky" else 0.0
-        with Lattice(19, nx, ny, nz, nd, storage=storage, omega=1.0 / 0.501, u_lbm=0.06,
+        with Lattice(q, nx, ny, nz, nd, storage=storage, omega=1.0 / 0.501, u_lbm=0.06,
                      cs=cs) as lat:
             if name == "network":
                 lat.set_closure(*split_closure(closure)[0])
@@ -748,7 +748,9 @@ def main(argv=None) -> int:
                     help="only compare the fine run in FP32 with the same run in FP64")
     ap.add_argument("--cost-only", default=None, metavar="NETWORK",
                     help="only time the saved network's collide against Smagorinsky "
-                         "(D3Q19 128x128x512, FP32 and FP16)")
+                         "(128x128x512, FP32 and FP16)")
+    ap.add_argument("--lattices", default="19",
+                    help="with --cost-only: the lattices to time, e.g. 15,19,27")
     a = ap.parse_args(argv)
     if a.precision:
         rep = precision_check(ClosureCase())
@@ -760,8 +762,11 @@ def main(argv=None) -> int:
         return 0
     if a.cost_only:
         net = load_network(a.cost_only)
-        cost = {f"128x128x512_{st}": step_cost((128, 128, 512), net, storage=st)
-                for st in ("f32", "f16")}
+        cost = {}
+        for q in (int(v) for v in a.lattices.split(",")):
+            prefix = "" if q == 19 else f"d3q{q}_"
+            for st in ("f32", "f16"):
+                cost[f"{prefix}128x128x512_{st}"] = step_cost((128, 128, 512), net, storage=st, q=q)
         print(json.dumps(cost, indent=1))
         return 0
     kw = {"snapshots": a.snapshots, "holdout": a.snapshots // 2}
