@@ -682,9 +682,19 @@ __device__ inline T stress_norm(const T (&pi)[6]) {
                 T(2) * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5]));
 }
 
-// apply_turbulence_model, LatticeStructure.hpp:140-153.
-template <class T>
+// apply_turbulence_model, LatticeStructure.hpp:140-153.  FAST (FP16
+// storage, whose own rounding is ~1e-3 of a population): the hardware square
+// roots and reciprocal, 1 ulp each, in place of the correctly rounded ones.
+template <class T, bool FAST = false>
 __device__ inline T smagorinsky_omega(const T (&pi)[6], const StepArgs &a) {
+    if constexpr (FAST) {
+        static_assert(sizeof(T) == 4, "FP32 arithmetic");
+        const float n2 = pi[0] * pi[0] + pi[1] * pi[1] + pi[2] * pi[2] +
+                         2.f * (pi[3] * pi[3] + pi[4] * pi[4] + pi[5] * pi[5]);
+        const float P = __builtin_amdgcn_sqrtf(__builtin_amdgcn_sqrtf(n2) * float(a.cs) + float(a.nu2)) -
+                        float(a.nu);
+        return __builtin_amdgcn_rcpf(3.f * (float(a.nu) + P * (1.f / 6.f)) + 0.5f);
+    }
     T P = stress_norm<T>(pi);
     P *= T(a.cs);
     P = sqrt(P + T(a.nu2)) - T(a.nu);
@@ -759,7 +769,7 @@ __device__ inline void node_update(T (&f)[L::Q], int t, T &rho, T &ux, T &uy, T 
     if (a.smag) {
         T pi[6];
         node_stress<L, T, SC>(f, feq, pi, w);
-        om = smagorinsky_omega<T>(pi, a);
+        om = smagorinsky_omega<T, SC>(pi, a);
     }
     if (a.dynamics == 1) {
         if constexpr (CONSERVE) {
