@@ -88,8 +88,7 @@ def test_training_converges_on_known_function(gpu_lib):
     assert last < first / 20, (first, last)
 
 
-def _channel():
-    nx, ny, nz = 16, 12, 24
+def _channel(nx=16, ny=12, nz=24):
     nd = geometry.channel(nx, ny, nz, obstacles=[geometry.sphere(7, 6, 8, 3)])
     return nx, ny, nz, nd
 
@@ -118,21 +117,24 @@ def test_constant_network_closure_equals_bgk(gpu_lib, storage, q):
     assert np.linalg.norm(u_a - u_b) / np.linalg.norm(u_b) < tol
 
 
+@pytest.mark.parametrize("shape", [(16, 12, 24), (13, 11, 19)])
 @pytest.mark.parametrize("q", [15, 19, 27])
 @pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_closure_run_follows_oracle(gpu_lib, storage, q):
+def test_closure_run_follows_oracle(gpu_lib, storage, q, shape):
     """The network closure in the collide kernel against the oracle's FP64
-    closure run (bf16 network roundings included) on a 16x12x24 channel
-    whose interior workgroups take the relative gather; FP16 storage adds
-    its own rounding (~1e-4 per step)."""
-    nx, ny, nz, nd = _channel()
+    closure run (bf16 network roundings included); FP16 storage adds its own
+    rounding (~1e-4 per step).  16x12x24 is nine whole 512-node workgroups
+    whose interior ones take the relative gather; 13x11x19 (2717 nodes) ends
+    in a partial workgroup, whose lanes past the last node run the network on
+    that node and store nothing."""
+    nx, ny, nz, nd = _channel(*shape)
     params = mlp_init(3)
     params[-1] = -1.0
     kw = dict(omega=1.0 / 0.51, u_lbm=0.06, rho_lbm=1.0)
     o = Oracle(q, nx, ny, nz, nd, **kw)
     o.set_closure(params, 40.0, 0.02)
     tol = 1e-4 if storage == "f32" else 2e-3
-    print(f"closure run q={q} {storage}", flush=True)
+    print(f"closure run q={q} {storage} {nx}x{ny}x{nz}", flush=True)
     with Lattice(q, nx, ny, nz, nd, storage=storage, **kw) as lat:
         lat.set_closure(params, 40.0, 0.02)
         for n in (2, 30):
