@@ -59,6 +59,30 @@ template <int ST> constexpr bool kClosureNTLoad = (ST == ST_F16);
 // (6).  FP32 D3Q27 stays at 4: at 5 it spills (325 against 299 us).
 template <int ST, int Q>
 constexpr int kClosureWaves = Q == 27 ? (ST == ST_F16 ? 5 : 4) : (ST == ST_F32 && Q == 19 ? 5 : 6);
+// Steps with kTypeAhead, and the closure collide in every storage, load the
+// chunk's common node type (or the node's own) ahead of their gathers, so
+// that it lands with them; loaded after them it costs each wavefront one
+// more memory round trip before the collision.  That pays where the step is
+// not HBM-bound: FP16 (128x128x512, us per step, same box: D3Q19 108.7 ->
+// 104.9, D3Q15 90.9 -> 82.9, D3Q27 155.5 -> 152.0) and the closure collide
+// (FP16 D3Q19 145.5 -> 141.5, D3Q15 127.1 -> 124.9, D3Q27 188.8 -> 189.7;
+// FP32 D3Q19 206.9 -> 205.4).  The FP32 and FP64 steps hide that round trip
+// already and ran 0.1-0.9 % slower with the load ahead.
+template <int ST> constexpr bool kTypeAhead = (ST == ST_F16);
+
+// The first of the node-type loads: the chunk's common type (0xFF where its
+// nodes differ) when the step has the chunk table, else the node's type.
+template <int MODE>
+__device__ inline uint32_t type_first(const uint8_t *nd_type, const uint8_t *chunk_type, uint32_t chunk,
+                                      uint32_t g) {
+    return (MODE == MODE_STEP && chunk_type) ? uint32_t(chunk_type[chunk]) : uint32_t(nd_type[g]);
+}
+
+// The node's type from the first load: the node's own where its chunk is mixed.
+template <int MODE>
+__device__ inline int type_final(uint32_t t0, const uint8_t *nd_type, const uint8_t *chunk_type, uint32_t g) {
+    return (MODE == MODE_STEP && chunk_type && t0 == 0xFFu) ? int(nd_type[g]) : int(t0);
+}
 
 // Launch arguments, passed by value (kernarg segment, scalar registers).
 struct StepArgs {
@@ -832,6 +856,9 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
     const uint32_t r = g - z * a.nxny;
     const uint32_t y = udiv(r, a.div_nx);
     const uint32_t x = r - y * a.nx;
+    const uint32_t chunk = (blk * BS) >> a.cb;  // BS divides the chunk
+    uint32_t t0 = 0;
+    if constexpr (kTypeAhead<ST>) t0 = type_first<MODE>(nd_type, chunk_type, chunk, g);
     T f[Q];
     if (MODE == MODE_STEP && a.row_chunks && pull_rel_ok<BS>(blk * BS, a)) {
         PullRow<L, ST> pr;
@@ -855,13 +882,8 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
     T w[Q];
 #pragma unroll
     for (int s = 0; s < Q; ++s) w[s] = S::weight(s, a);
-    int t;
-    if (MODE == MODE_STEP && chunk_type) {
-        const uint8_t ct = chunk_type[(blk * BS) >> a.cb];  // BS divides the chunk
-        t = (ct != 0xFF) ? ct : nd_type[g];
-    } else {
-        t = nd_type[g];
-    }
+    if constexpr (!kTypeAhead<ST>) t0 = type_first<MODE>(nd_type, chunk_type, chunk, g);
+    const int t = type_final<MODE>(t0, nd_type, chunk_type, g);
     T rho, ux, uy, uz;
     if (MODE == MODE_STEP) {
         node_update<L, T, S::kShift, true, kConserve<ST, Q>, S::kScaled>(f, t, rho, ux, uy, uz, a,
@@ -947,6 +969,8 @@ k_lbm_closure(const typename StorageT<ST>::st *__restrict__ fin, typename Storag
     const uint32_t r = g - z * a.nxny;
     const uint32_t y = udiv(r, a.div_nx);
     const uint32_t x = r - y * a.nx;
+    // the node type is loaded ahead of the gathers, in every storage
+    const uint32_t t0 = type_first<MODE_STEP>(nd_type, chunk_type, (blockIdx.x * BS) >> a.cb, g);
     T f[Q];
     if (a.row_chunks && pull_rel_ok<BS>(blockIdx.x * BS, a)) {
         PullRow<L, ST> pr;
@@ -970,13 +994,7 @@ k_lbm_closure(const typename StorageT<ST>::st *__restrict__ fin, typename Storag
     T w[Q];
 #pragma unroll
     for (int s = 0; s < Q; ++s) w[s] = S::weight(s, a);
-    int t;
-    if (chunk_type) {
-        const uint8_t ct = chunk_type[(blockIdx.x * BS) >> a.cb];
-        t = (ct != 0xFF) ? ct : nd_type[g];
-    } else {
-        t = nd_type[g];
-    }
+    const int t = type_final<MODE_STEP>(t0, nd_type, chunk_type, g);
     T feq[Q], rho, ux, uy, uz;
     const bool relaxes = node_prepare<L, T, S::kShift, true, S::kScaled>(f, feq, t, rho, ux, uy, uz,
                                                                          a, uy_head, w, g);
