@@ -25,8 +25,12 @@ Smagorinsky's constant cs in {0.025, 0.05, 0.1} and the same base offsets
     inlet, run beside it, moves Smagorinsky's figure by only 2e-8: the
     coarse long run itself is not sensitive, the trained network is.
 
+* cost: the network's FP32 step takes at most 1.3x the Smagorinsky step,
+  on the coarse lattice and on the 128x128x512 benchmark lattice.
+
 The run is deterministic (seeded inlet, seeded sampling, deterministic
-kernels), so the margins are fixed numbers, not draws.
+kernels), so the margins are fixed numbers, not draws; the cost ratios are
+timings (measured 1.01-1.14).
 """
 import numpy as np
 import pytest
@@ -78,3 +82,13 @@ def test_held_out_mean_flow_matches_calibrated_smagorinsky(gpu_lib, pipeline):
     print("calibrated smagorinsky", smag, "network", net, "smagorinsky, inlet perturbed", noise)
     assert "diverged" not in net
     assert abs(net["mean_l2"] / smag["mean_l2"] - 1.0) < 0.05
+
+
+def test_network_step_cost_within_bound(gpu_lib, pipeline):
+    """Per-step cost of the trained network's collide against the
+    Smagorinsky collide, FP32 storage, HIP events, best of 3 x 200 steps."""
+    _, _, _, cl, _, _ = pipeline
+    for shape in ((128, 64, 64), (128, 128, 512)):
+        cost = closure.step_cost(shape, cl, storage="f32")
+        print(shape, cost, flush=True)
+        assert cost["ratio"] <= 1.3, (shape, cost)
