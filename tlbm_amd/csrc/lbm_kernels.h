@@ -69,6 +69,16 @@ constexpr int kClosureWaves = Q == 27 ? (ST == ST_F16 ? 5 : 4) : (ST == ST_F32 &
 // FP32 D3Q19 206.9 -> 205.4).  The FP32 and FP64 steps hide that round trip
 // already and ran 0.1-0.9 % slower with the load ahead.
 template <int ST> constexpr bool kTypeAhead = (ST == ST_F16);
+// The step raises its wavefront's priority (s_setprio 3) while it issues
+// its gathers and drops it once they are issued, so that wavefronts still
+// issuing loads win the arbiter over those computing.  Where it pays is a
+// matter of measurement (128x128x512, us per step, same box, two runs each):
+// D3Q27 FP32 292.8 -> 289.4, FP16 151.4 -> 146.8, FP64 594.2 -> 587.3; D3Q19
+// FP16 104.9 -> 103.0.  It cost D3Q19 FP32 (201.8 -> 205.1), D3Q19 FP64
+// (401.0 -> 418.2) and D3Q15 FP16 (81.9 -> 84.2), and left D3Q15 FP32/FP64
+// unchanged, so those keep the default priority.  A second box confirmed
+// it, FP32X D3Q27 included (291.9 -> 289.4).
+template <int ST, int Q> constexpr bool kGatherPrio = (Q == 27) || (ST == ST_F16 && Q == 19);
 
 // The first of the node-type loads: the chunk's common type (0xFF where its
 // nodes differ) when the step has the chunk table, else the node's type.
@@ -843,6 +853,7 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
     using st = typename S::st;
     constexpr int Q = L::Q;
     if (MODE == MODE_STEP) load_args_early(a, fin);
+    if constexpr (MODE == MODE_STEP && kGatherPrio<ST, Q>) __builtin_amdgcn_s_setprio(3);
     uint32_t g, blk = 0;
     if (MODE == MODE_HEAD) {
         g = threadIdx.x;
@@ -879,6 +890,7 @@ k_lbm(const typename StorageT<ST>::st *__restrict__ fin, typename StorageT<ST>::
             f[s] = S::in(NTLOAD ? __builtin_nontemporal_load(p) : *p, s, a);
         }
     }
+    if constexpr (MODE == MODE_STEP && kGatherPrio<ST, Q>) __builtin_amdgcn_s_setprio(0);
     T w[Q];
 #pragma unroll
     for (int s = 0; s < Q; ++s) w[s] = S::weight(s, a);
