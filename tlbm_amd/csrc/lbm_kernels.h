@@ -278,9 +278,11 @@ constexpr uint32_t kStreamStep = 0x9E3779B9u;
 //   FP32 D3Q27 forced from 4 to 5 waves, 2.057 against 2.043 ms.
 // * FP32: 256-thread workgroups (D3Q19: 203 against 205 us with 512; D3Q27
 //   porous: 2.05 ms, the same as 512 and 1024).
-// * FP16: 512-thread workgroups.
-// FP32 and FP16 use plain loads: FP16 109.7 against 111.5 us per step with
-// non-temporal loads; FP32 D3Q19 202.9 against 205.0, D3Q27 2.05 against
+// * FP16: 256-thread workgroups, since its node-type load went ahead of the
+//   gathers (D3Q19 101.6 against 103.1 us with 512, D3Q27 145.0 against
+//   147.0; D3Q15 82.3 against 82.2).
+// FP32 and FP16 use plain loads: FP16 101.6 against 105.4 us per step with
+// non-temporal loads (D3Q27 145.0 against 153.8); FP32 D3Q19 202.9 against 205.0, D3Q27 2.05 against
 // 2.17 ms.  Every kernel stores
 // non-temporally.
 template <int ST, int Q> struct Tune;
@@ -297,7 +299,7 @@ template <int Q> struct Tune<ST_F32X, Q> {
     static constexpr bool kNTLoad = false;
 };
 template <int Q> struct Tune<ST_F16, Q> {
-    static constexpr int kBS = 512;
+    static constexpr int kBS = 256;
     static constexpr bool kNTLoad = false;
 };
 
