@@ -47,7 +47,12 @@ enum Mode : int { MODE_STEP = 0, MODE_MACROS = 1, MODE_HEAD = 2, MODE_AVG = 3, M
 constexpr int kBlock = 256;  // 4 wave64 per workgroup
 constexpr int kRed = 7;      // values per FP64 reduction partial
 constexpr int kFilterPlanes = 16;  // box filter output planes
-constexpr int kClosureBlock = 512;  // workgroup of the closure collide
+// workgroup of the closure collide.  With the weights staged by direct
+// global-to-LDS loads, 256 threads beat 512 (128x128x512, us per step, same
+// box: FP16 D3Q19 137.9 -> 132.6, D3Q15 121.6 -> 118.5, D3Q27 184.3 ->
+// 177.5; FP32 D3Q19 205.1 -> 201.9, D3Q15 162.3 -> 160.1, D3Q27 291.6 ->
+// 289.7): a smaller workgroup waits at the weights' barrier for fewer waves.
+constexpr int kClosureBlock = 256;
 // non-temporal population loads in the closure collide: FP16 only (FP32:
 // 221.8 against 217.6 us per step with them)
 template <int ST> constexpr bool kClosureNTLoad = (ST == ST_F16);
