@@ -83,7 +83,7 @@ def _lane_parts(g, g0, dims, ks, cb, ch, kb):
 ])
 @pytest.mark.parametrize("storage,bs", [("f64", 1024), ("f64", 512), ("f32", 256), ("f16", 256),
                                         ("f16", 512), ("f32", 512)])
-@pytest.mark.parametrize("q", [19, 27])
+@pytest.mark.parametrize("q", [15, 19, 27])
 def test_relative_gathers_land_on_the_source(dims, planes, storage, bs, q):
     nx, ny, nz, nxny, n, ks, cb, ch, kb, rel, fbytes = _setup(dims, q, storage)
     if not rel or ch % bs:
